@@ -1,0 +1,185 @@
+/*
+ * su3gf.h -- C ABI of the MI355X (gfx950) SU(3) gauge-fixing library.
+ *
+ * Plain C: raw device pointers, sizes and status codes; no C++ or torch types.
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference's src/ directory).  The C++ headers in include/culgt/ keep
+ * the reference's class names and forward to these functions.
+ *
+ * Link storage: component `comp` of link (site idx, direction mu) lives at
+ *   U[idx + V*(comp + stride*mu)],  comp = c + 2*(j + 3*i)
+ * with idx the parity-split site index (even sites first), i/j row/column,
+ * c = 0 real / 1 imaginary.  stride = 18 is the reference's GpuPattern layout
+ * (lattice/access_pattern/GpuPattern.hxx:53-55); stride = 12 stores rows 0,1
+ * only.  dims = {Nt, Nx, Ny, Nz}, mu = 0 is time, every extent even.
+ * A Coulomb timeslice buffer is a lattice with dims {1, Nx, Ny, Nz}.
+ *
+ * Every entry returns a gf_status; gf_last_error() describes the last failure
+ * on the calling thread.  `stream` is a hipStream_t (NULL = default stream).
+ */
+#ifndef SU3GF_H_
+#define SU3GF_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GF_OK = 0,
+    GF_ERR_BAD_SHAPE = 1,     /* odd or non-positive extent, size mismatch */
+    GF_ERR_BAD_ARG = 2,       /* null pointer, unknown enum, bad parameter */
+    GF_ERR_HIP = 3,           /* a HIP runtime call failed */
+    GF_ERR_NOT_CONVERGED = 4, /* a driver hit its iteration limit */
+    GF_ERR_NO_DEVICE = 5      /* no usable gfx950 device */
+} gf_status;
+
+/* lattice/datatype/lattice_typedefs.h:39 (enum GaugeType) */
+typedef enum { GF_LANDAU = 0, GF_COULOMB = 1, GF_MAG = 2, GF_U1XU1 = 3 } gf_gauge;
+
+/* gaugefixing/algorithms/: OrUpdate, MicroUpdate, SaUpdate, SrUpdate, RandomUpdate */
+typedef enum { GF_OR = 0, GF_MICRO = 1, GF_SA = 2, GF_SR = 3, GF_RANDOM = 4 } gf_algo;
+
+typedef enum { GF_F32 = 0, GF_F64 = 1 } gf_dtype;
+
+typedef enum { GF_LAYOUT_R12 = 12, GF_LAYOUT_PATTERN18 = 18 } gf_layout;
+
+/* lattice/datatype/lattice_typedefs.h:42 (enum StoppingCrit) */
+typedef enum { GF_CRIT_MAX = 0, GF_CRIT_AVERAGE = 1 } gf_crit;
+
+typedef struct {
+    int dims[4];    /* {Nt, Nx, Ny, Nz} */
+    int dtype;      /* gf_dtype */
+    int layout;     /* gf_layout */
+} gf_lattice;
+
+/* ---------------------------------------------------------------- status */
+const char *gf_last_error(void);
+const char *gf_version(void);
+
+/* ------------------------------------------------- device and memory
+ * Thin wrappers over the HIP runtime so that callers without a HIP binding
+ * (ctypes, cgo, JNI) can own device buffers.  These replace the apps'
+ * cudaMalloc / cudaMemcpy / cudaFree calls
+ * (gaugefixing/apps/LandauGaugeFixingSU3_4D.cu:98-116,189,280). */
+gf_status gf_device_info(int *count, char *name, size_t name_len);
+gf_status gf_malloc(void **ptr, size_t bytes);
+gf_status gf_free(void *ptr);
+gf_status gf_memcpy_h2d(void *dst, const void *src, size_t bytes);
+gf_status gf_memcpy_d2h(void *dst, const void *src, size_t bytes);
+gf_status gf_memcpy_d2d(void *dst, const void *src, size_t bytes);
+gf_status gf_memset(void *ptr, int value, size_t bytes);
+gf_status gf_synchronize(void);
+gf_status gf_stream_create(void **stream);
+gf_status gf_stream_destroy(void *stream);
+gf_status gf_stream_synchronize(void *stream);
+gf_status gf_event_create(void **event);
+gf_status gf_event_destroy(void *event);
+gf_status gf_event_record(void *event, void *stream);
+gf_status gf_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* Bytes of one link array for a lattice (V * 4 * layout * sizeof(real)). */
+size_t gf_lattice_bytes(const gf_lattice *lat);
+
+/* ---------------------------------------------------- operator API
+ * One parity half-sweep with update rule `algo`.
+ * Replaces LandauKernelsSU3::{orStep,microStep,saStep,randomTrafo}
+ *   (gaugefixing/LandauKernelsSU3.hxx:55-105, kernels :216-286),
+ * MAGKernelsSU3::{orStep,microStep,saStep,srStep,randomTrafo}
+ *   (gaugefixing/MAGKernelsSU3.hxx:57-110, kernels :248-327),
+ * U1xU1KernelsSU3::{orStep,microStep,saStep,randomTrafo}
+ *   (gaugefixing/U1xU1KernelsSU3.hxx:55-105, kernels :216-286), and for
+ * gauge = GF_COULOMB, CoulombKernelsSU3::{orStep,microStep,saStep,randomTrafo}
+ *   (gaugefixing/CoulombKernelsSU3.hxx:53-101, kernels :208-279), where U is
+ * the timeslice t buffer (UtUp), Udw the slice t-1 buffer (UtDw) and dims
+ * {1, Nx, Ny, Nz}.  Udw is ignored for the other gauges.
+ * param: OR/SR parameter or SA temperature (a float, as in the reference).
+ * seed, counter: the Philox key word and launch counter of the RNG updates
+ * (SA, SR, RANDOM); nsb: the reference's sites-per-block, which enters the RNG
+ * key (LandauKernelsSU3.hxx:272); 0 selects the reference default 32.
+ * The neighbour table the reference passes is computed on the fly. */
+gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int gauge, int algo,
+                   float param, int seed, uint32_t counter, int nsb, void *stream);
+
+/* Reunitarize every link and, for the 18-real layout, write its third row.
+ * Replaces CommonKernelsSU3::projectSU3 (gaugefixing/CommonKernelsSU3.hxx:40-62,71-90). */
+gf_status gf_project(const gf_lattice *lat, void *U, void *stream);
+
+/* Random SU(3) links (product of three random SU(2) subgroup elements).
+ * Replaces CommonKernelsSU3::setHot (gaugefixing/CommonKernelsSU3.hxx:92-130). */
+gf_status gf_set_hot(const gf_lattice *lat, void *U, int seed, uint32_t counter, void *stream);
+
+/* Copy links between layouts / precisions (rows 0,1; the third row is
+ * reconstructed for an 18-real destination).  reunitarize != 0 projects each
+ * link in the destination precision.  No reference counterpart: it moves a
+ * configuration between the reference layout and the compact one. */
+gf_status gf_convert(const gf_lattice *src_lat, const void *src, const gf_lattice *dst_lat, void *dst,
+                     int reunitarize, void *stream);
+
+/* ------------------------------------------------------ quality / stats
+ * Replaces GaugeFixingStats<Ndim,Nc,GType,StoppingCrit>
+ * (gaugefixing/GaugeFixingStats.hxx:55-99,186-232): owns the reduction
+ * scratch for one lattice shape.  gauge selects the quality kernel
+ * (Landau/U1xU1: LandauKernelsSU3.hxx:114-191; Coulomb: CoulombKernelsSU3.hxx:110-183;
+ * MAG: MAGKernelsSU3.hxx:116-223). */
+typedef struct gf_stats gf_stats;
+
+gf_status gf_stats_create(const int dims[4], int gauge, int crit, gf_stats **out);
+gf_status gf_stats_destroy(gf_stats *st);
+
+/* generateGaugeQuality + getCurrentGff/getCurrentA: gff and theta with the
+ * reference's prefactors (gff/(Nc*Ndim*V) -- Coulomb Nc*(Ndim-1) --, theta/Nc,
+ * and /V for GF_CRIT_AVERAGE). */
+gf_status gf_stats_quality(gf_stats *st, const gf_lattice *lat, const void *U, double *gff, double *theta,
+                           void *stream);
+
+/* generateGaugeQualityPerSite (e.g. LandauKernelsSU3.hxx:71-74): per-site
+ * gff_x and theta_x into caller-owned device arrays of V doubles each. */
+gf_status gf_quality_per_site(const gf_lattice *lat, int gauge, const void *U, double *dGff, double *dA,
+                              void *stream);
+
+/* ---------------------------------------------------------- drivers */
+typedef struct {
+    int sweeps;          /* OR sweeps done (per phase summed) */
+    int sweeps_fp32;     /* of which in FP32 (mixed driver) */
+    int checks;          /* quality evaluations */
+    double gff;          /* last evaluated functional */
+    double theta;        /* last evaluated theta */
+    double ms_total;     /* device time of the whole run (hipEvents) */
+    double ms_fp32;      /* device time of the FP32 phase (mixed driver) */
+    int converged;       /* theta < precision reached */
+} gf_run_result;
+
+/* OR loop with the reference's control semantics
+ * (gaugefixing/apps/LandauGaugeFixingSU3_4D.cu:244-262): for i < maxiter:
+ * two parity sweeps; projectSU3 when i % reproject == 0; a quality check when
+ * i % checkprecision == 0 that stops once theta < precision. */
+gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
+                    int checkprecision, int reproject, gf_run_result *out, void *stream);
+
+/* Mixed precision: U (FP64, layout of lat) is copied into a compact FP32
+ * working array, OR-swept there until theta < switch_precision, reunitarized
+ * back into FP64 and finished in FP64 down to precision.  The FP64 working
+ * array is U itself.  No reference counterpart (the reference's DPUPDATES is
+ * a compile-time FP32/FP64 mix, gaugefixing/apps/Makefile:46-48). */
+gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,
+                       float precision, int maxiter, int checkprecision, int reproject, gf_run_result *out,
+                       void *stream);
+
+/* --------------------------------------------------- synthetic inputs
+ * Host-side generator of the benchmark configurations (not a reference
+ * interface): links U = exp(i sum_a c_a lambda_a / 2), c_a ~ N(0, sigma^2)
+ * (Philox-keyed by seed and link), then, when haar_seed != 0, a Haar-random
+ * SU(3) gauge transformation g(x) (Philox-keyed by haar_seed and site):
+ * U_mu(x) <- g(x) U_mu(x) g(x+mu)^dagger.  Writes all 18 reals in the
+ * GpuPattern layout into host memory U (V*72 doubles).  sigma = 0 gives unit
+ * links (a pure gauge after the transformation). */
+gf_status gf_make_config(const int dims[4], double sigma, uint32_t seed, uint32_t haar_seed, double *U);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SU3GF_H_ */

@@ -1,0 +1,79 @@
+"""Build the gfx950 library ``libsu3gf.so`` in-tree.
+
+HIP sources are compiled by hipcc for gfx950 only, with packed-FP32 code
+generation disabled; the host-only configuration generator is compiled by g++
+with OpenMP.  Objects go to ``_build/`` next to this file, the shared library
+to ``libsu3gf.so`` beside it.
+
+Usage: ``python -m su3_ma_gauge_gpu_amd.build [--verbose]``
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libsu3gf.so"
+INCLUDE = PKG.parent / "include"
+
+HIP_SOURCES = ["su3gf_kernels.hip", "su3gf_api.hip"]
+HOST_SOURCES = ["su3gf_config.cpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-result",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    f"-I{INCLUDE}",
+]
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", f"-I{INCLUDE}"]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:2])} ... ({res.returncode})")
+    if verbose and (res.stdout or res.stderr):
+        print(res.stdout + res.stderr, flush=True)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source and link ``libsu3gf.so``; returns its path."""
+    hipcc = _hipcc()
+    BUILD.mkdir(exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = BUILD / (src + ".o")
+        _run([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        objs.append(str(obj))
+    for src in HOST_SOURCES:
+        obj = BUILD / (src + ".o")
+        _run(["g++", *GXX_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="--verbose" in sys.argv))

@@ -1,0 +1,220 @@
+"""Parity of the gfx950 kernels with the CPU oracle (FP64), called through the
+C ABI via the Python mirror of the reference's classes.
+
+Tolerances: the GPU sums the 8 link contributions of a site as a two-level
+butterfly and contracts multiply-adds into FMAs, the oracle sums in the
+reference's ladder order without contraction; per half-sweep the link
+components therefore agree to a few ulp (tolerances below are 1e-12 absolute
+on components of modulus <= 1).  Transcendental RNG updates (SA, random
+transform) go through OCML on the GPU and glibc in the oracle; the Philox
+streams themselves are identical.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import COULOMB, LANDAU, MAG, MICRO, OR, RANDOM, SA, SR, U1XU1, rows01
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 4, 4, 4), (4, 6, 4, 8), (6, 4, 8, 4)]
+KERNELS = {LANDAU: "LandauKernelsSU3", MAG: "MAGKernelsSU3", U1XU1: "U1xU1KernelsSU3"}
+
+
+def _config(gf, dims, sigma=0.3, seed=1, haar=2):
+    return gf.make_config(dims, sigma, seed, haar)
+
+
+def _run_parity(gf, oracle, gauge, algo, dims, nsweeps, param, seed=7, counter0=11):
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims)
+    U0 = _config(gf, dims)
+    Uo = U0.copy()
+    d = gf.DeviceBuffer.from_numpy(U0)
+    K = getattr(gf, KERNELS[gauge])(lat)
+    a, b = V // 2 // 32, 256
+    counter = counter0
+    for _ in range(nsweeps):
+        for parity in (0, 1):
+            if algo == OR:
+                K.orStep(a, b, d, None, parity, param)
+            elif algo == MICRO:
+                K.microStep(a, b, d, None, parity)
+            elif algo == SA:
+                K.saStep(a, b, d, None, parity, param, seed, counter)
+            elif algo == SR:
+                K.srStep(a, b, d, None, parity, param, seed, counter)
+            else:
+                K.randomTrafo(a, b, d, None, parity, seed, counter)
+            oracle.sweep(gauge, algo, dims, Uo, parity, param, seed, counter)
+            counter += 1
+    Ug = d.download(np.float64, U0.size)
+    return U0, Uo, Ug, V
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+@pytest.mark.parametrize("gauge", [LANDAU, MAG, U1XU1])
+@pytest.mark.parametrize("algo", [OR, MICRO])
+def test_sweep_deterministic_updates(gf, gpu, oracle, gauge, algo, dims):
+    # The microcanonical update (omega = 2 reflection) is not contractive: an
+    # ulp-level difference grows about tenfold per half-sweep even between two
+    # oracle runs, so it is compared after one sweep; OR after three.
+    nsweeps = 3 if algo == OR else 1
+    U0, Uo, Ug, V = _run_parity(gf, oracle, gauge, algo, dims, nsweeps=nsweeps, param=1.7)
+    err = np.abs(rows01(Ug, V) - rows01(Uo, V)).max()
+    assert err < 1e-12, f"rows 0,1 differ by {err}"
+    # the sweep never writes the third row (LandauKernelsSU3.hxx:255)
+    assert np.array_equal(Ug.reshape(4, 18, V)[:, 12:, :], U0.reshape(4, 18, V)[:, 12:, :])
+    assert not np.array_equal(rows01(Ug, V), rows01(U0, V))
+
+
+@pytest.mark.parametrize("gauge", [LANDAU, U1XU1])
+def test_micro_preserves_functional(gf, gpu, gauge):
+    """A microcanonical step replaces the local functional's maximiser direction
+    by its reflection, so the Landau functional is unchanged (exactly, in exact
+    arithmetic) however many sweeps are done."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims)
+    d = gf.DeviceBuffer.from_numpy(_config(gf, dims))
+    st = gf.GaugeFixingStats(gf.LandauKernelsSU3, d, lat)
+    st.generateGaugeQuality()
+    g0, t0 = st.getCurrentGff(), st.getCurrentA()
+    K = getattr(gf, KERNELS[gauge])(lat)
+    for _ in range(10):
+        for parity in (0, 1):
+            K.microStep(V // 2 // 32, 256, d, None, parity)
+    st.generateGaugeQuality()
+    assert st.getCurrentGff() == pytest.approx(g0, rel=1e-12, abs=1e-13)
+    assert st.getCurrentA() != pytest.approx(t0, rel=1e-3)  # the links did move
+
+
+@pytest.mark.parametrize("gauge", [LANDAU, MAG, U1XU1])
+@pytest.mark.parametrize("algo,param", [(SA, 0.4), (RANDOM, 0.0)] + [(SR, 0.5)])
+def test_sweep_rng_updates(gf, gpu, oracle, gauge, algo, param):
+    if algo == SR and gauge != MAG:
+        pytest.skip("stochastic relaxation is a MAG update in the reference")
+    dims = (4, 6, 4, 8)
+    U0, Uo, Ug, V = _run_parity(gf, oracle, gauge, algo, dims, nsweeps=2, param=param)
+    err = np.abs(rows01(Ug, V) - rows01(Uo, V)).max()
+    assert err < 1e-10, f"rows 0,1 differ by {err}"
+
+
+@pytest.mark.parametrize("algo,param", [(OR, 1.7), (MICRO, 0.0), (SA, 0.7), (RANDOM, 0.0)])
+def test_coulomb_slice_sweep(gf, gpu, oracle, algo, param):
+    dims3 = (1, 8, 6, 4)
+    V = int(np.prod(dims3))
+    lat = gf.Lattice(dims3)
+    up0 = _config(gf, dims3, seed=3, haar=4)
+    dw0 = _config(gf, dims3, seed=5, haar=6)
+    upo, dwo = up0.copy(), dw0.copy()
+    dup, ddw = gf.DeviceBuffer.from_numpy(up0), gf.DeviceBuffer.from_numpy(dw0)
+    K = gf.CoulombKernelsSU3(lat)
+    a, b = V // 2 // 32, 256
+    for it in range(3):
+        for parity in (0, 1):
+            c = 100 + 2 * it + parity
+            if algo == OR:
+                K.orStep(a, b, dup, ddw, None, parity, param)
+            elif algo == MICRO:
+                K.microStep(a, b, dup, ddw, None, parity)
+            elif algo == SA:
+                K.saStep(a, b, dup, ddw, None, parity, param, 9, c)
+            else:
+                K.randomTrafo(a, b, dup, ddw, None, parity, 9, c)
+            oracle.sweep_coulomb(algo, dims3, upo, dwo, parity, param, 9, c)
+    upg, dwg = dup.download(np.float64, up0.size), ddw.download(np.float64, dw0.size)
+    assert np.abs(rows01(upg, V) - rows01(upo, V)).max() < 1e-10
+    assert np.abs(rows01(dwg, V) - rows01(dwo, V)).max() < 1e-10
+    # only the time links of slice t-1 are touched
+    assert np.array_equal(dwg.reshape(4, 18, V)[1:], dw0.reshape(4, 18, V)[1:])
+
+
+@pytest.mark.parametrize("gauge", [LANDAU, MAG, U1XU1])
+def test_quality(gf, gpu, oracle, gauge):
+    dims = (4, 6, 4, 8)
+    U = _config(gf, dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    lat = gf.Lattice(dims)
+    K = getattr(gf, KERNELS[gauge])
+    st = gf.GaugeFixingStats(K, d, lat)
+    st.generateGaugeQuality()
+    g_ref, t_ref = oracle.quality(gauge, dims, U)
+    assert st.getCurrentGff() == pytest.approx(g_ref, rel=1e-13)
+    assert st.getCurrentA() == pytest.approx(t_ref, rel=1e-12)
+    # per-site values
+    V = int(np.prod(dims))
+    dg, da = gf.DeviceBuffer(8 * V), gf.DeviceBuffer(8 * V)
+    K(lat).generateGaugeQualityPerSite(V // 32, 32, d, dg, da)
+    go, ao = oracle.quality_per_site(gauge, dims, U)
+    assert np.allclose(dg.download(), go, rtol=1e-12, atol=1e-13)
+    assert np.allclose(da.download(), ao, rtol=1e-11, atol=1e-13)
+
+
+def test_quality_coulomb_and_max(gf, gpu, oracle):
+    dims3 = (1, 8, 6, 4)
+    U = _config(gf, dims3, seed=3, haar=4)
+    d = gf.DeviceBuffer.from_numpy(U)
+    lat = gf.Lattice(dims3)
+    st = gf.GaugeFixingStats(gf.CoulombKernelsSU3, d, lat)
+    st.generateGaugeQuality()
+    g_ref, t_ref = oracle.quality(COULOMB, dims3, U)
+    assert st.getCurrentGff() == pytest.approx(g_ref, rel=1e-13)
+    assert st.getCurrentA() == pytest.approx(t_ref, rel=1e-12)
+    # StoppingCrit MAX: theta = max_x theta_x / Nc
+    st_max = gf.GaugeFixingStats(gf.CoulombKernelsSU3, d, lat, crit=gf.CRIT_MAX)
+    st_max.generateGaugeQuality()
+    _, ao = oracle.quality_per_site(COULOMB, dims3, U)
+    assert st_max.getCurrentA() == pytest.approx(ao.max() / 3.0, rel=1e-12)
+
+
+def test_project_and_set_hot(gf, gpu, oracle):
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims)
+    U = _config(gf, dims)
+    # perturb rows 0,1 off the group so that projection has work to do
+    U = U + 1e-3 * np.random.default_rng(0).standard_normal(U.size)
+    d = gf.DeviceBuffer.from_numpy(U)
+    gf.CommonKernelsSU3(lat).projectSU3(V // 32, 32, d)
+    Uo = U.copy()
+    oracle.project_all(dims, Uo)
+    assert np.abs(d.download() - Uo).max() < 1e-13
+    h = gf.DeviceBuffer(lat.nbytes)
+    gf.CommonKernelsSU3(lat).setHot(V // 32, 32, h, None, 5, 17)
+    Uh = np.zeros(V * 72)
+    oracle.set_hot(dims, Uh, 5, 17)
+    assert np.abs(h.download() - Uh).max() < 1e-13
+
+
+def test_convert_roundtrip(gf, gpu):
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    lat64 = gf.Lattice(dims)
+    lat32 = gf.Lattice(dims, gf.F32, gf.R12)
+    U = _config(gf, dims)
+    d64 = gf.DeviceBuffer.from_numpy(U)
+    d32 = gf.DeviceBuffer(lat32.nbytes)
+    back = gf.DeviceBuffer(lat64.nbytes)
+    import ctypes as C
+    gf._lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(d64.ptr), C.byref(lat32.struct),
+                 C.c_void_p(d32.ptr), 0, None)
+    f = d32.download(np.float32, lat32.count).reshape(4, 12, V)
+    assert np.array_equal(f, rows01(U, V).astype(np.float32))
+    gf._lib.call("gf_convert", C.byref(lat32.struct), C.c_void_p(d32.ptr), C.byref(lat64.struct),
+                 C.c_void_p(back.ptr), 1, None)
+    Ub = back.download()
+    assert np.abs(Ub - U).max() < 1e-6
+
+
+def test_run_or_config0_matches_oracle(gf, gpu, oracle):
+    """8^4 Landau, sigma 0.3 seed 1, Haar seed 2, omega 1.7, theta < 1e-12,
+    checked every sweep, reprojected every 100 (including sweep 0)."""
+    dims = (8, 8, 8, 8)
+    U = _config(gf, dims, 0.3, 1, 2)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_or(gf.Lattice(dims), d, gf.LANDAU, 1.7, 1e-12, 5000, 1, 100)
+    Uo = U.copy()
+    n, g, t = oracle.run_or(LANDAU, dims, Uo, 1.7, 1e-12, 5000, 1, 100)
+    assert res.converged and t < 1e-12
+    assert abs(res.sweeps - n) <= 1
+    assert res.gff == pytest.approx(g, rel=1e-10)

@@ -159,13 +159,39 @@ typedef struct {
 gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
                     int checkprecision, int reproject, gf_run_result *out, void *stream);
 
-/* Mixed precision: U (FP64, layout of lat) is copied into a compact FP32
- * working array, OR-swept there until theta < switch_precision, reunitarized
- * back into FP64 and finished in FP64 down to precision.  The FP64 working
- * array is U itself.  No reference counterpart (the reference's DPUPDATES is
- * a compile-time FP32/FP64 mix, gaugefixing/apps/Makefile:46-48). */
+/* OR to theta < precision with the quality check fused into the sweep
+ * kernels: theta of the odd sites is formed after the odd update, theta of
+ * the even sites before the next even update, and the last block of each
+ * kernel finishes the reduction in a fixed order, sets a device flag on
+ * convergence and makes every later sweep of the run exit at once.  The host
+ * reads the flag once per 32 sweeps.  Every sweep is checked (the reference's
+ * checkprecision = 1); out->sweeps is the index + 1 of the first sweep whose
+ * end state met the precision and out->gff / out->theta are that state's; the
+ * array then holds that state plus one even half-sweep.  projectSU3 after
+ * sweep i when i % reproject == 0.  No reference counterpart (the reference
+ * checks with a separate kernel and a host copy,
+ * gaugefixing/GaugeFixingStats.hxx:186-232). */
+gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
+                      int reproject, gf_run_result *out, void *stream);
+
+/* Average duration of one fused-check sweep kernel (the kernels gf_run_fast
+ * and gf_run_mixed launch), from HIP events around nsweeps back-to-back
+ * OR sweeps on `stream` (after two warm-up sweeps).  with_gauge selects the
+ * FP32 variant that also accumulates a gauge transformation.  U is modified. */
+gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int with_gauge,
+                         float *ms_per_kernel, void *stream);
+
+/* Mixed precision.  U (FP64, layout of lat) holds the configuration U0 and
+ * is left untouched during an FP32 phase that sweeps a compact FP32 copy and
+ * accumulates each site's gauge transformation g(x) in FP32; every `refresh`
+ * sweeps the FP32 links are rebuilt as G U0 G^dagger from U0 and the FP64
+ * projection G of g, so FP32 rounding never alters the gauge orbit.  Once
+ * theta < switch_precision, U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 and
+ * fused-check FP64 sweeps continue to theta < precision.  No reference
+ * counterpart (the reference's DPUPDATES is a compile-time FP32/FP64 mix,
+ * gaugefixing/apps/Makefile:46-48). */
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,
-                       float precision, int maxiter, int checkprecision, int reproject, gf_run_result *out,
+                       float precision, int maxiter, int refresh, int reproject, gf_run_result *out,
                        void *stream);
 
 /* --------------------------------------------------- synthetic inputs

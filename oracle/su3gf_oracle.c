@@ -807,11 +807,11 @@ ORC_EXPORT int orc_quality(int gauge, const int dims[4], const Real *U, double *
 }
 
 /* ------------------------------------------------------------------ */
-/* Driver loop (LandauGaugeFixingSU3_4D.cu:244-262): OR sweeps with       */
+/* Driver loop (LandauGaugeFixingSU3_4D.cu:244-271): OR sweeps with       */
 /* reprojection when i % reproject == 0 and a check when                */
 /* i % checkprecision == 0 that stops once theta < precision (a float). */
 /* Returns the number of completed sweeps; *gff / *theta hold the last   */
-/* check's values.                                                        */
+/* check's values.  A final projectSU3 writes the third rows.            */
 /* ------------------------------------------------------------------ */
 ORC_EXPORT int orc_run_or(int gauge, const int dims[4], Real *U, float omega, float precision,
                           int maxiter, int checkprecision, int reproject, double *gff, double *theta)
@@ -829,6 +829,8 @@ ORC_EXPORT int orc_run_or(int gauge, const int dims[4], Real *U, float omega, fl
             if (t < precision) break;
         }
     }
+    /* reconstruct the third rows (LandauGaugeFixingSU3_4D.cu:271) */
+    orc_project_all(dims, U);
     *gff = g;
     *theta = t;
     return done;

@@ -47,7 +47,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_mixed", "device_info", "RunResult",
+    "make_config", "run_or", "run_fast", "run_mixed", "device_info", "RunResult",
 ]
 
 
@@ -56,6 +56,8 @@ def _ptr(x) -> C.c_void_p:
         return C.c_void_p(None)
     if isinstance(x, DeviceBuffer):
         return C.c_void_p(x.ptr)
+    if isinstance(x, C.c_void_p):
+        return x
     return C.c_void_p(int(x))
 
 
@@ -385,13 +387,25 @@ def run_or(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, precisi
     return _result(r)
 
 
+def run_fast(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, precision: float = 1e-12,
+             maxiter: int = 20000, reproject: int = 100, stream=None) -> RunResult:
+    """OR with the check fused into the sweep kernels (every sweep checked,
+    no host synchronisation per sweep); see gf_run_fast in include/su3gf.h."""
+    r = _lib.gf_run_result()
+    _lib.call("gf_run_fast", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(precision),
+              int(maxiter), int(reproject), C.byref(r), _ptr(stream))
+    return _result(r)
+
+
 def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,
-              precision: float = 1e-12, maxiter: int = 10000, checkprecision: int = 1, reproject: int = 100,
+              precision: float = 1e-12, maxiter: int = 20000, refresh: int = 50, reproject: int = 100,
               stream=None) -> RunResult:
-    """FP32 compact-layout OR to ``switch_precision``, then FP64 to ``precision``."""
+    """FP32 sweeps with an accumulated gauge transformation down to
+    ``switch_precision``, the transformation applied to the FP64 configuration,
+    then FP64 fused-check sweeps to ``precision``."""
     r = _lib.gf_run_result()
     _lib.call("gf_run_mixed", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(switch_precision),
-              C.c_float(precision), int(maxiter), int(checkprecision), int(reproject), C.byref(r), _ptr(stream))
+              C.c_float(precision), int(maxiter), int(refresh), int(reproject), C.byref(r), _ptr(stream))
     return _result(r)
 
 

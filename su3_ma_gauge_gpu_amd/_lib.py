@@ -88,6 +88,8 @@ SIGNATURES = {
     "gf_stats_quality": (_i, [_vp, _pl, _vp, C.POINTER(_d), C.POINTER(_d), _vp]),
     "gf_quality_per_site": (_i, [_pl, _i, _vp, _vp, _vp, _vp]),
     "gf_run_or": (_i, [_pl, _vp, _i, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_run_fast": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_make_config": (_i, [_pi4, _d, _u32, _u32, C.POINTER(_d)]),
 }

@@ -21,7 +21,13 @@ BUILD = PKG / "_build"
 LIB = PKG / "libsu3gf.so"
 INCLUDE = PKG.parent / "include"
 
-HIP_SOURCES = ["su3gf_kernels.hip", "su3gf_api.hip"]
+# (source, extra flags): the exact unit keeps every multiply and add separate
+HIP_SOURCES = [
+    ("su3gf_exact.hip", ["-ffp-contract=off"]),
+    ("su3gf_fast.hip", []),
+    ("su3gf_kernels.hip", []),
+    ("su3gf_api.hip", []),
+]
 HOST_SOURCES = ["su3gf_config.cpp"]
 
 HIPCC_FLAGS = [
@@ -45,6 +51,12 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
 
 
+def _start(cmd: list[str], verbose: bool) -> subprocess.Popen:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
 def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -60,15 +72,25 @@ def build(verbose: bool = False) -> Path:
     """Compile every source and link ``libsu3gf.so``; returns its path."""
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
-    objs = []
-    for src in HIP_SOURCES:
+    objs, jobs = [], []
+    for src, extra in HIP_SOURCES:
         obj = BUILD / (src + ".o")
-        _run([hipcc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        jobs.append(_start([hipcc, *HIPCC_FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)], verbose))
         objs.append(str(obj))
     for src in HOST_SOURCES:
         obj = BUILD / (src + ".o")
-        _run(["g++", *GXX_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose)
+        jobs.append(_start(["g++", *GXX_FLAGS, "-c", str(CSRC / src), "-o", str(obj)], verbose))
         objs.append(str(obj))
+    failed = []
+    for job in jobs:
+        out, _ = job.communicate()
+        if job.returncode != 0:
+            failed.append(out)
+        elif verbose and out:
+            print(out, flush=True)
+    if failed:
+        sys.stderr.write("\n".join(failed))
+        raise RuntimeError("compilation failed")
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"], verbose)
     os.replace(tmp, LIB)

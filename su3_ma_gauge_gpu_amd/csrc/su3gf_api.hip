@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 
 #include "su3gf_kernels.h"
@@ -239,7 +240,7 @@ gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int ga
     const Geom g = make_geom(lat->dims, V);
     hipError_t e;
     if (lat->dtype == GF_F64) {
-        SweepArgs<double> a;
+        SweepArgs<double> a{};
         a.U = (double *)U;
         a.Udw = gauge == GF_COULOMB ? (double *)Udw : (double *)U;
         a.g = g;
@@ -248,9 +249,9 @@ gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int ga
         a.seed = (uint32_t)seed;
         a.counter = counter;
         a.nsb = nsb;
-        e = launch_sweep<double>(lat->layout, gauge, algo, a, S(stream));
+        e = exact::launch_sweep<double>(lat->layout, gauge, algo, a, S(stream));
     } else {
-        SweepArgs<float> a;
+        SweepArgs<float> a{};
         a.U = (float *)U;
         a.Udw = gauge == GF_COULOMB ? (float *)Udw : (float *)U;
         a.g = g;
@@ -259,7 +260,7 @@ gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int ga
         a.seed = (uint32_t)seed;
         a.counter = counter;
         a.nsb = nsb;
-        e = launch_sweep<float>(lat->layout, gauge, algo, a, S(stream));
+        e = exact::launch_sweep<float>(lat->layout, gauge, algo, a, S(stream));
     }
     if (e != hipSuccess) return hip_fail(e, "sweep kernel launch");
     return GF_OK;
@@ -270,8 +271,8 @@ gf_status gf_project(const gf_lattice *lat, void *U, void *stream)
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U) return fail(GF_ERR_BAD_ARG, "U is null");
-    hipError_t e = lat->dtype == GF_F64 ? launch_project<double>(lat->layout, (double *)U, V, S(stream))
-                                        : launch_project<float>(lat->layout, (float *)U, V, S(stream));
+    hipError_t e = lat->dtype == GF_F64 ? exact::launch_project<double>(lat->layout, 4, (double *)U, V, S(stream))
+                                        : exact::launch_project<float>(lat->layout, 4, (float *)U, V, S(stream));
     if (e != hipSuccess) return hip_fail(e, "project kernel launch");
     return GF_OK;
 }
@@ -282,8 +283,8 @@ gf_status gf_set_hot(const gf_lattice *lat, void *U, int seed, uint32_t counter,
     GF_TRY(check_lattice(lat, &V));
     if (!U) return fail(GF_ERR_BAD_ARG, "U is null");
     if (lat->layout != GF_LAYOUT_PATTERN18) return fail(GF_ERR_BAD_ARG, "set_hot writes the 18-real layout");
-    hipError_t e = lat->dtype == GF_F64 ? launch_set_hot<double>((double *)U, V, (uint32_t)seed, counter, S(stream))
-                                        : launch_set_hot<float>((float *)U, V, (uint32_t)seed, counter, S(stream));
+    hipError_t e = lat->dtype == GF_F64 ? exact::launch_set_hot<double>((double *)U, V, (uint32_t)seed, counter, S(stream))
+                                        : exact::launch_set_hot<float>((float *)U, V, (uint32_t)seed, counter, S(stream));
     if (e != hipSuccess) return hip_fail(e, "set_hot kernel launch");
     return GF_OK;
 }
@@ -318,7 +319,7 @@ gf_status gf_stats_create(const int dims[4], int gauge, int crit, gf_stats **out
     st->gauge = gauge;
     st->crit = crit;
     st->V = V;
-    st->nblocks = quality_blocks(V);
+    st->nblocks = exact::quality_blocks(V);
     hipError_t e = hipMalloc((void **)&st->partial, sizeof(double) * 2 * st->nblocks);
     if (e == hipSuccess) e = hipMalloc((void **)&st->out, sizeof(double) * 2);
     if (e == hipSuccess) e = hipHostMalloc((void **)&st->host, sizeof(double) * 2, hipHostMallocDefault);
@@ -355,10 +356,10 @@ static gf_status quality_launch(const gf_lattice *lat, int gauge, const void *U,
     hipError_t e;
     if (lat->dtype == GF_F64) {
         QualityArgs<double> a{(const double *)U, g, dGff, dA, partial, crit_max};
-        e = launch_quality<double>(lat->layout, gauge, a, s);
+        e = exact::launch_quality<double>(lat->layout, gauge, a, s);
     } else {
         QualityArgs<float> a{(const float *)U, g, dGff, dA, partial, crit_max};
-        e = launch_quality<float>(lat->layout, gauge, a, s);
+        e = exact::launch_quality<float>(lat->layout, gauge, a, s);
     }
     if (e != hipSuccess) return hip_fail(e, "quality kernel launch");
     return GF_OK;
@@ -445,6 +446,8 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
     if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) rc = fail(GF_ERR_HIP, "event creation");
     if (rc == GF_OK && hipEventRecord(ev.a, s) != hipSuccess) rc = fail(GF_ERR_HIP, "event record");
     if (rc == GF_OK) rc = or_phase(lat, U, gauge, omega, precision, maxiter, checkprecision, reproject, st, s, out);
+    // reconstruct the third rows (LandauGaugeFixingSU3_4D.cu:271)
+    if (rc == GF_OK) rc = gf_project(lat, U, s);
     if (rc == GF_OK && hipEventRecord(ev.b, s) != hipSuccess) rc = fail(GF_ERR_HIP, "event record");
     if (rc == GF_OK) {
         float ms = 0;
@@ -456,8 +459,217 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
     return rc;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------- fused-check drivers
+namespace {
+
+struct FusedCtx {
+    DevState *st = nullptr;     // device
+    DevState *host = nullptr;   // pinned
+    double *partial = nullptr;  // 2 * nblocks doubles for each parity
+    double *even() { return partial; }
+    double *odd() { return partial + 2 * nblocks; }
+    int64_t nblocks = 0;
+    ~FusedCtx()
+    {
+        if (st) (void)hipFree(st);
+        if (host) (void)hipHostFree(host);
+        if (partial) (void)hipFree(partial);
+    }
+    gf_status init(int64_t Vh)
+    {
+        nblocks = sweep_blocks(Vh);
+        HIP_TRY(hipMalloc((void **)&st, sizeof(DevState)));
+        HIP_TRY(hipHostMalloc((void **)&host, sizeof(DevState), hipHostMallocDefault));
+        HIP_TRY(hipMalloc((void **)&partial, sizeof(double) * 4 * nblocks));  // even, odd
+        return GF_OK;
+    }
+    gf_status read(hipStream_t s)
+    {
+        HIP_TRY(hipMemcpyAsync(host, st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return GF_OK;
+    }
+};
+
+constexpr int kChunk = 32;  // sweeps enqueued between two host reads of the device state
+
+double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12.; }
+
+// One OR phase with the check fused into the sweeps.  `after_sweep(it)` runs
+// after sweep `it` (reprojection, refresh).  Sweeps after convergence exit at
+// once.  On return r->theta / r->gff describe the state after the first sweep
+// that met `tol` (the buffer then holds that state plus one even half-sweep),
+// or the final state if maxiter was reached.
+template <typename R, typename After>
+gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
+                      FusedCtx &fc, hipStream_t s, After after_sweep, int *sweeps_done, gf_run_result *r)
+{
+    int64_t V = 1;
+    for (int i = 0; i < 4; i++) V *= lat->dims[i];
+    const Geom g = make_geom(lat->dims, V);
+    hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, s);
+    if (e != hipSuccess) return hip_fail(e, "state init");
+    SweepArgs<R> a{};
+    a.U = U;
+    a.Udw = U;
+    a.gacc = gacc;
+    a.state = fc.st;
+    a.g = g;
+    a.param = omega;
+    a.nsb = 32;
+    int it = 0;
+    bool done = false;
+    while (it < maxiter && !done) {
+        const int n = std::min(kChunk, maxiter - it);
+        for (int k = 0; k < n; k++, it++) {
+            a.sweep = it;
+            a.parity = 0;
+            a.partial = fc.even();
+            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, 1, gacc != nullptr, a, s);
+            if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
+            if (it > 0) {
+                e = launch_fused_check(fc.st, fc.even(), fc.odd(), fc.nblocks, it, s);
+                if (e != hipSuccess) return hip_fail(e, "check kernel launch");
+            }
+            a.parity = 1;
+            a.partial = fc.odd();
+            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, 2, gacc != nullptr, a, s);
+            if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
+            GF_TRY(after_sweep(it));
+        }
+        GF_TRY(fc.read(s));
+        done = fc.host->done != 0;
+    }
+    r->checks += fc.host->checks;
+    if (done) {
+        *sweeps_done = fc.host->converged_sweep + 1;
+        r->theta = fc.host->theta;
+        r->gff = fc.host->gff;
+        r->converged = 1;
+    } else {
+        *sweeps_done = it;
+        r->converged = 0;
+    }
+    return GF_OK;
+}
+
+gf_status final_quality(const gf_lattice *lat, const void *U, int gauge, gf_run_result *r, hipStream_t s)
+{
+    gf_stats *st = nullptr;
+    GF_TRY(gf_stats_create(lat->dims, gauge == GF_COULOMB ? GF_LANDAU : gauge, GF_CRIT_AVERAGE, &st));
+    gf_status rc = gf_stats_quality(st, lat, U, &r->gff, &r->theta, s);
+    gf_stats_destroy(st);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
+                      int reproject, gf_run_result *out, void *stream)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
+    if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_run_fast drives 4D gauges");
+    if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
+        return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
+    memset(out, 0, sizeof *out);
+    hipStream_t s = S(stream);
+    FusedCtx fc;
+    GF_TRY(fc.init(V / 2));
+    EventPair ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    HIP_TRY(hipEventRecord(ev.a, s));
+    auto after = [&](int it) -> gf_status {
+        if (reproject > 0 && it % reproject == 0) return gf_project(lat, U, s);
+        return GF_OK;
+    };
+    int done = 0;
+    gf_status rc = lat->dtype == GF_F64
+                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, fc, s, after,
+                                             &done, out)
+                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, fc, s, after,
+                                            &done, out);
+    GF_TRY(rc);
+    out->sweeps = done;
+    GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
+    HIP_TRY(hipEventRecord(ev.b, s));
+    float ms = 0;
+    HIP_TRY(hipEventSynchronize(ev.b));
+    HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    out->ms_total = ms;
+    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, out, s));
+    return GF_OK;
+}
+
+gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int with_gauge, float *ms_per_kernel,
+                         void *stream)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
+    if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D gauges");
+    if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
+        return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
+    if (with_gauge && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "gauge accumulation is an FP32 variant");
+    hipStream_t s = S(stream);
+    FusedCtx fc;
+    GF_TRY(fc.init(V / 2));
+    struct G {
+        float *p = nullptr;
+        ~G() { if (p) (void)hipFree(p); }
+    } gb;
+    if (with_gauge) {
+        HIP_TRY(hipMalloc((void **)&gb.p, sizeof(float) * 12 * V));
+        HIP_TRY(launch_gauge_identity(gb.p, V, s));
+    }
+    // tol = 0: no state ever converges, every kernel does its full work
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, s));
+    EventPair ev;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    auto run = [&](int n, int first) -> gf_status {
+        for (int it = 0; it < n; it++) {
+            if (lat->dtype == GF_F64) {
+                SweepArgs<double> a{};
+                a.U = (double *)U; a.Udw = a.U; a.state = fc.st;
+                a.g = make_geom(lat->dims, V); a.param = 1.7f; a.nsb = 32; a.sweep = first + it;
+                for (int par = 0; par < 2; par++) {
+                    a.parity = par;
+                    a.partial = par ? fc.odd() : fc.even();
+                    HIP_TRY(fast::launch_sweep<double>(lat->layout, gauge, GF_OR, par + 1, false, a, s));
+                }
+            } else {
+                SweepArgs<float> a{};
+                a.U = (float *)U; a.Udw = a.U; a.gacc = gb.p; a.state = fc.st;
+                a.g = make_geom(lat->dims, V); a.param = 1.7f; a.nsb = 32; a.sweep = first + it;
+                for (int par = 0; par < 2; par++) {
+                    a.parity = par;
+                    a.partial = par ? fc.odd() : fc.even();
+                    HIP_TRY(fast::launch_sweep<float>(12, gauge, GF_OR, par + 1, with_gauge != 0, a, s));
+                }
+            }
+        }
+        return GF_OK;
+    };
+    GF_TRY(run(2, 0));  // warm-up
+    HIP_TRY(hipEventRecord(ev.a, s));
+    GF_TRY(run(nsweeps, 2));
+    HIP_TRY(hipEventRecord(ev.b, s));
+    float ms = 0;
+    HIP_TRY(hipEventSynchronize(ev.b));
+    HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    *ms_per_kernel = ms / (2.0f * nsweeps);
+    return GF_OK;
+}
+
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision, float precision,
-                       int maxiter, int checkprecision, int reproject, gf_run_result *out, void *stream)
+                       int maxiter, int refresh, int reproject, gf_run_result *out, void *stream)
 {
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
@@ -465,45 +677,77 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "the mixed driver takes an FP64 configuration");
     if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_run_mixed drives 4D gauges");
     memset(out, 0, sizeof *out);
+    hipStream_t s = S(stream);
     gf_lattice l32 = *lat;
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
-    void *U32 = nullptr;
-    HIP_TRY(hipMalloc(&U32, gf_lattice_bytes(&l32)));
-    gf_stats *st = nullptr;
-    gf_status rc = gf_stats_create(lat->dims, gauge, GF_CRIT_AVERAGE, &st);
+    const Geom geo = make_geom(lat->dims, V);
+    struct Bufs {
+        void *U32 = nullptr, *g = nullptr;
+        ~Bufs()
+        {
+            if (U32) (void)hipFree(U32);
+            if (g) (void)hipFree(g);
+        }
+    } b;
+    HIP_TRY(hipMalloc(&b.U32, gf_lattice_bytes(&l32)));
+    HIP_TRY(hipMalloc(&b.g, sizeof(float) * 12 * V));
+    FusedCtx fc;
+    GF_TRY(fc.init(V / 2));
     EventPair ev;
     hipEvent_t mid = nullptr;
-    hipStream_t s = S(stream);
-    if (rc == GF_OK && (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess ||
-                        hipEventCreate(&mid) != hipSuccess))
-        rc = fail(GF_ERR_HIP, "event creation");
-    if (rc == GF_OK && hipEventRecord(ev.a, s) != hipSuccess) rc = fail(GF_ERR_HIP, "event record");
-    if (rc == GF_OK) rc = gf_convert(lat, U, &l32, U32, 0, s);
-    if (rc == GF_OK)
-        rc = or_phase(&l32, U32, gauge, omega, switch_precision, maxiter, checkprecision, reproject, st, s, out);
-    out->sweeps_fp32 = out->sweeps;
-    // back to FP64 with a reunitarization in FP64
-    if (rc == GF_OK) rc = gf_convert(&l32, U32, lat, U, 1, s);
-    if (rc == GF_OK && hipEventRecord(mid, s) != hipSuccess) rc = fail(GF_ERR_HIP, "event record");
-    if (rc == GF_OK) {
-        out->converged = 0;
-        const int left = maxiter - out->sweeps;
-        rc = or_phase(lat, U, gauge, omega, precision, left, checkprecision, reproject, st, s, out);
-    }
-    if (rc == GF_OK && hipEventRecord(ev.b, s) != hipSuccess) rc = fail(GF_ERR_HIP, "event record");
-    if (rc == GF_OK) {
-        float ms = 0, ms1 = 0;
-        if (hipEventSynchronize(ev.b) != hipSuccess || hipEventElapsedTime(&ms, ev.a, ev.b) != hipSuccess ||
-            hipEventElapsedTime(&ms1, ev.a, mid) != hipSuccess)
-            rc = fail(GF_ERR_HIP, "event timing");
-        out->ms_total = ms;
-        out->ms_fp32 = ms1;
-    }
-    if (mid) (void)hipEventDestroy(mid);
-    gf_stats_destroy(st);
-    (void)hipFree(U32);
-    return rc;
+    HIP_TRY(hipEventCreate(&ev.a));
+    HIP_TRY(hipEventCreate(&ev.b));
+    HIP_TRY(hipEventCreate(&mid));
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } mg{mid};
+    float *g = (float *)b.g;
+    HIP_TRY(hipEventRecord(ev.a, s));
+    HIP_TRY(launch_gauge_identity(g, V, s));
+    HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
+    // FP32 phase: links and gauge transformation in FP32; every `refresh`
+    // sweeps the working links are rebuilt from the FP64 configuration and
+    // the (reprojected) transformation, which removes the FP32 drift of the
+    // gauge-invariant content.
+    auto after32 = [&](int it) -> gf_status {
+        const int k = it + 1;
+        if (refresh > 0 && k % refresh == 0) {
+            HIP_TRY(launch_project_gauge(g, V, s));
+            HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, b.U32, GF_F32, 12, geo, s));
+        } else if (reproject > 0 && it % reproject == 0) {
+            HIP_TRY(exact::launch_project<float>(12, 4, (float *)b.U32, V, s));
+        }
+        return GF_OK;
+    };
+    int done32 = 0;
+    GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, fc, s, after32, &done32,
+                              out));
+    out->sweeps_fp32 = done32;
+    // switch: apply the accumulated transformation to the FP64 configuration
+    HIP_TRY(launch_project_gauge(g, V, s));
+    HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, U, GF_F64, lat->layout, geo, s));
+    HIP_TRY(hipEventRecord(mid, s));
+    auto after64 = [&](int it) -> gf_status {
+        if (reproject > 0 && it % reproject == 0) return gf_project(lat, U, s);
+        return GF_OK;
+    };
+    int done64 = 0;
+    out->converged = 0;
+    const int left = std::max(0, maxiter - done32);
+    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, fc, s, after64, &done64, out));
+    out->sweeps = done32 + done64;
+    GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
+    HIP_TRY(hipEventRecord(ev.b, s));
+    float ms = 0, ms1 = 0;
+    HIP_TRY(hipEventSynchronize(ev.b));
+    HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+    HIP_TRY(hipEventElapsedTime(&ms1, ev.a, mid));
+    out->ms_total = ms;
+    out->ms_fp32 = ms1;
+    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, out, s));
+    return GF_OK;
 }
 
 }  // extern "C"

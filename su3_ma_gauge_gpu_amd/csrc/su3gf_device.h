@@ -211,6 +211,14 @@ __device__ __forceinline__ int64_t backward(const Geom &g, const int c[4], int p
     return site_index(g, d, par ^ 1);
 }
 
+// forward neighbour x + mu
+__device__ __forceinline__ int64_t forward(const Geom &g, const int c[4], int par, int mu)
+{
+    int d[4] = {c[0], c[1], c[2], c[3]};
+    d[mu] = (d[mu] + 1 == g.N[mu]) ? 0 : d[mu] + 1;
+    return site_index(g, d, par ^ 1);
+}
+
 // ---------------------------------------------------------------- Philox
 struct Philox {
     uint32_t k0, k1, c0, c1;

@@ -1,18 +1,37 @@
 // su3gf_kernels.h -- kernel argument blocks and launchers shared by the
-// kernel translation unit and the C ABI implementation.
+// kernel translation units and the C ABI implementation.
 #pragma once
 #include "su3gf_device.h"
 #include "../../include/su3gf.h"
 
 namespace su3gf {
 
-constexpr int kBlock = 256;  // 4 waves of 64 lanes = 64 sites per block
+constexpr int kBlock = 256;  // 4 waves of 64 lanes: one wave per direction, 64 sites per block
+
+// Device-resident state of a fused-theta driver run.  Written by the check
+// kernel that follows each even sweep kernel; read by the host after a chunk.
+struct DevState {
+    int done;             // theta < tol reached
+    int converged_sweep;  // index of the first sweep whose end state met tol
+    int checked_sweep;    // index of the sweep whose end state `theta` describes
+    int checks;           // number of completed checks
+    int pad0_, pad1_;
+    double tol;           // theta threshold
+    double theta_scale;   // 1 / (Nc * V)
+    double gff_scale;     // 1 / (Nc * Ndim * V)
+    double theta;         // theta of sweep `checked_sweep` (prefactors applied)
+    double gff;           // gff of the same state
+};
 
 template <typename R> struct SweepArgs {
     R *U;          // links (timeslice t for Coulomb)
     R *Udw;        // Coulomb: timeslice t-1 buffer; otherwise == U
+    float *gacc;   // gauge transformation accumulated by the FP32 phase (or null)
+    DevState *state;   // fused-theta state (or null)
+    double *partial;   // per-block (theta, gff) partials of this kernel (fused check)
     Geom g;
     int parity;
+    int sweep;     // sweep index (fused check)
     float param;   // OR/SR parameter or SA temperature
     uint32_t seed, counter;
     int64_t nsb;   // sites per block of the reference launch (RNG key)
@@ -26,13 +45,38 @@ template <typename R> struct QualityArgs {
     int crit_max;
 };
 
+inline int64_t sweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
+
+// exact translation unit (reference order, no FMA contraction)
+namespace exact {
 template <typename R> hipError_t launch_sweep(int stride, int gauge, int algo, const SweepArgs<R> &a, hipStream_t s);
-template <typename R> hipError_t launch_project(int stride, R *U, int64_t V, hipStream_t s);
+template <typename R> hipError_t launch_project(int stride, int nfields, R *U, int64_t V, hipStream_t s);
 template <typename R> hipError_t launch_set_hot(R *U, int64_t V, uint32_t seed, uint32_t counter, hipStream_t s);
 template <typename R> hipError_t launch_quality(int stride, int gauge, const QualityArgs<R> &a, hipStream_t s);
+int64_t quality_blocks(int64_t V);
+}  // namespace exact
+
+// fast translation unit (contracted, fused theta, gauge accumulation)
+namespace fast {
+// theta: 0 none, 1 pre-update (even kernel), 2 post-update (odd kernel)
+template <typename R>
+hipError_t launch_sweep(int stride, int gauge, int algo, int theta, bool gacc, const SweepArgs<R> &a, hipStream_t s);
+}  // namespace fast
+
+// utility kernels (su3gf_kernels.hip)
 hipError_t launch_convert(int ti, int si, const void *src, int to, int so, void *dst, int64_t V, int reun,
                           hipStream_t s);
 hipError_t launch_reduce_final(const double *partial, int64_t n, int crit_max, double *out, hipStream_t s);
-int64_t quality_blocks(int64_t V);
+hipError_t launch_gauge_identity(float *g, int64_t V, hipStream_t s);
+hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s);
+// out_mu(x) = G(x) U0_mu(x) G(x+mu)^dagger with G = FP64 projection of g(x);
+// U0 FP64 in layout s0; out of type `to` in layout so (may alias U0 if same type/layout)
+hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *out, int to, int so, const Geom &geo,
+                              hipStream_t s);
+hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, hipStream_t s);
+// theta of the state after sweep `sweep - 1` from the even partials of sweep
+// `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update)
+hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int sweep,
+                              hipStream_t s);
 
 }  // namespace su3gf

@@ -1,0 +1,451 @@
+// su3gf_sweep.h -- the checkerboard SU(2)-subgroup sweep kernel for gfx950.
+//
+// Included by two translation units: su3gf_exact.hip (compiled with
+// -ffp-contract=off, EXACT = true: every multiply and add rounds separately,
+// in the reference's order, and 1/sqrt replaces rsqrt) and su3gf_fast.hip
+// (EXACT = false: FMA contraction, hardware rsqrt, optional fused theta and
+// gauge-transformation accumulation).
+//
+// Work mapping: a 256-thread block is four waves; wave w handles direction
+// mu = w and lane l handles site p = 64*blockIdx + l of the parity, so every
+// load instruction reads 64 consecutive sites of one link component (512 B in
+// FP64).  Lane (mu, p) owns the up link U_mu(x) and the down link U_mu(x-mu).
+// Per SU(2) subgroup each lane writes the contributions of its two links to
+// LDS; after one barrier every lane of the site reads the eight contributions
+// and adds them in the reference's order -- up links mu = 0..3, then down
+// links mu = 0..3 (GaugeFixingSubgroupStep.hxx:126-208) -- so the four waves
+// hold bit-identical sums and compute the same update, which each lane
+// applies to its own two links.  Two LDS buffers alternate between subgroups,
+// so one barrier per subgroup suffices.
+//
+// Reference (paths relative to its src/): GaugeFixingSubgroupStep.hxx:74-705,
+// algorithms/{Or,Micro,Sa,Sr,Random}Update.hxx, {Landau,Coulomb,MAG,U1xU1}KernelsSU3.hxx
+// apply(), LandauKernelsSU3.hxx:114-191 (theta), CommonKernelsSU3.hxx:71-90.
+#pragma once
+#include "su3gf_kernels.h"
+
+namespace su3gf {
+
+template <bool EXACT, typename R> __device__ __forceinline__ R inv_sqrt(R v)
+{
+    if (EXACT) return (R)1 / sqrt_(v);
+    return rsq(v);
+}
+
+// third row = conj(row0 x row1) / |.|, written as the reference does
+// (SU3.hxx:655-664)
+template <bool EXACT, typename R> __device__ __forceinline__ void reconstruct_e(M3<R> &u)
+{
+    const Cx<R> a = conj(u.e[0][1]) * conj(u.e[1][2]) - conj(u.e[0][2]) * conj(u.e[1][1]);
+    const Cx<R> b = conj(u.e[0][2]) * conj(u.e[1][0]) - conj(u.e[0][0]) * conj(u.e[1][2]);
+    const Cx<R> c = conj(u.e[0][0]) * conj(u.e[1][1]) - conj(u.e[0][1]) * conj(u.e[1][0]);
+    const R n = inv_sqrt<EXACT>(abs2(a) + abs2(b) + abs2(c));
+    u.e[2][0] = a * n;
+    u.e[2][1] = b * n;
+    u.e[2][2] = c * n;
+}
+
+// projectSU3 (SU3.hxx:536-583): normalise row 0, orthogonalise and normalise
+// row 1 (divisions as in the reference), rebuild row 2
+template <bool EXACT, typename R> __device__ __forceinline__ void project_e(M3<R> &u)
+{
+    R abs_u = abs2(u.e[0][0]);
+    abs_u = abs_u + abs2(u.e[0][1]);
+    abs_u = abs_u + abs2(u.e[0][2]);
+    abs_u = sqrt_(abs_u);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.e[0][k] = cx(u.e[0][k].x / abs_u, u.e[0][k].y / abs_u);
+    Cx<R> sp = u.e[1][0] * conj(u.e[0][0]);
+    sp = sp + u.e[1][1] * conj(u.e[0][1]);
+    sp = sp + u.e[1][2] * conj(u.e[0][2]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.e[1][k] = u.e[1][k] - u.e[0][k] * sp;
+    R abs_v = abs2(u.e[1][0]);
+    abs_v = abs_v + abs2(u.e[1][1]);
+    abs_v = abs_v + abs2(u.e[1][2]);
+    abs_v = sqrt_(abs_v);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.e[1][k] = cx(u.e[1][k].x / abs_v, u.e[1][k].y / abs_v);
+    reconstruct_e<EXACT>(u);
+}
+
+// ------------------------------------------------------------ update rules
+__device__ __forceinline__ double cospi_(double x) { return cospi(x); }
+__device__ __forceinline__ float cospi_(float x) { return cospif(x); }
+__device__ __forceinline__ double sinpi_(double x) { return sinpi(x); }
+__device__ __forceinline__ float sinpi_(float x) { return sinpif(x); }
+__device__ __forceinline__ double log_(double x) { return log(x); }
+__device__ __forceinline__ float log_(float x) { return logf(x); }
+
+// OrUpdate.hxx:52-78
+template <bool E, typename R> __device__ __forceinline__ void upd_or(R a[4], float omega)
+{
+    const R ai = a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+    const R a0 = a[0] * a[0];
+    const R b = ((R)omega * a0 + ai) / (a0 + ai);
+    const R c = inv_sqrt<E>(a0 + b * b * ai);
+    a[0] *= c;
+    a[1] *= b * c;
+    a[2] *= b * c;
+    a[3] *= b * c;
+}
+
+// MicroUpdate.hxx:43-66
+template <bool E, typename R> __device__ __forceinline__ void upd_micro(R a[4])
+{
+    const R ai = a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+    const R a0 = a[0] * a[0];
+    if (E) {
+        const R b = (R)2 * a[0] / (a0 + ai);
+        a[0] = (a0 - ai) / (a0 + ai);
+        a[1] *= b;
+        a[2] *= b;
+        a[3] *= b;
+    } else {
+        const R inv = (R)1 / (a0 + ai);
+        const R b = (R)2 * a[0] * inv;
+        a[0] = (a0 - ai) * inv;
+        a[1] *= b;
+        a[2] *= b;
+        a[3] *= b;
+    }
+}
+
+// SrUpdate.hxx:54-96
+template <bool E, typename R> __device__ __forceinline__ void upd_sr(R a[4], float p, Philox &rng)
+{
+    const R r = philox_rand(rng, R(0));
+    const R a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+    if (r < (R)p) {
+        a[0] = a0 * a0 - a1 * a1 - a2 * a2 - a3 * a3;
+        a[1] = (R)2 * a0 * a1;
+        a[2] = (R)2 * a0 * a2;
+        a[3] = (R)2 * a0 * a3;
+    }
+    const R c = inv_sqrt<E>(a[0] * a[0] + a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+    a[0] *= c;
+    a[1] *= c;
+    a[2] *= c;
+    a[3] *= c;
+}
+
+// RandomUpdate.hxx:47-59
+template <typename R> __device__ __forceinline__ void upd_random(R a[4], Philox &rng)
+{
+    const R alpha = philox_rand(rng, R(0));
+    const R phi = (R)2 * philox_rand(rng, R(0));
+    const R cos_theta = (R)2 * philox_rand(rng, R(0)) - (R)1;
+    const R sin_theta = sqrt_((R)1 - cos_theta * cos_theta);
+    const R sin_alpha = sinpi_(alpha);
+    a[0] = cospi_(alpha);
+    a[1] = sin_alpha * sin_theta * cospi_(phi);
+    a[2] = sin_alpha * sin_theta * sinpi_(phi);
+    a[3] = sin_alpha * cos_theta;
+}
+
+// SaUpdate.hxx:52-147 (Kennedy-Pendleton heatbath, temperature = 1/beta)
+template <bool E, typename R> __device__ __forceinline__ void upd_sa(R a[4], float temperature, Philox &rng)
+{
+    R e0 = a[0], e1 = -a[1], e2 = -a[2], e3 = -a[3];
+    const R dk = inv_sqrt<E>(e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3);
+    const R p0 = dk * (R)temperature;
+    R delta;
+    for (;;) {
+        R r1, r2;
+        do { r1 = philox_rand(rng, R(0)); } while (r1 < (R)0.0001);
+        r1 = -log_(r1) * p0;
+        do { r2 = philox_rand(rng, R(0)); } while (r2 < (R)0.0001);
+        r2 = -log_(r2) * p0;
+        R r3 = cospi_((R)2 * philox_rand(rng, R(0)));
+        r3 = r3 * r3;
+        delta = r2 + r1 * r3;
+        const R r4 = philox_rand(rng, R(0));
+        if (!(r4 * r4 > (R)1 - (R)0.5 * delta)) break;
+    }
+    const R b0 = (R)1 - delta;
+    const R cos_theta = (R)2 * philox_rand(rng, R(0)) - (R)1;
+    const R sin_theta = sqrt_((R)1 - cos_theta * cos_theta);
+    const R sin_alpha = sqrt_((R)1 - b0 * b0);
+    const R phi = (R)2 * philox_rand(rng, R(0));
+    const R b1 = sin_alpha * sin_theta * cospi_(phi);
+    const R b2 = sin_alpha * sin_theta * sinpi_(phi);
+    const R b3 = sin_alpha * cos_theta;
+    e0 *= dk;
+    e1 *= dk;
+    e2 *= dk;
+    e3 *= dk;
+    a[0] = b0 * e0 + b3 * e3 + b2 * e2 + e1 * b1;
+    a[3] = e0 * b3 - e3 * b0 + b1 * e2 - b2 * e1;
+    a[2] = b3 * e1 - b0 * e2 + b2 * e0 - b1 * e3;
+    a[1] = b2 * e3 + b1 * e0 - b3 * e2 - e1 * b0;
+}
+
+template <typename R, int A, bool E> __device__ __forceinline__ void calc_update(R a[4], float param, Philox &rng)
+{
+    if (A == GF_OR) upd_or<E>(a, param);
+    else if (A == GF_MICRO) upd_micro<E>(a);
+    else if (A == GF_SA) upd_sa<E>(a, param, rng);
+    else if (A == GF_SR) upd_sr<E>(a, param, rng);
+    else upd_random(a, rng);
+}
+
+// ---------------------------------------------------- per-link contributions
+// Landau / Coulomb / U1xU1: the raw subgroup quaternion q(U) (SU3.hxx:193-217).
+// MAG: D = |a00|^2 - |a01|^2 - |a10|^2 + |a11|^2 and c = a00 conj(a10) - a01 conj(a11)
+// (up) or conj(a00) a01 - conj(a10) a11 (down), stored as (D, Im c, Re c)
+// (GaugeFixingSubgroupStep.hxx:387-390,457-460).
+template <typename R, int G, int I, int J, bool UP>
+__device__ __forceinline__ void link_contrib(const M3<R> &u, R v[4])
+{
+    if (G == GF_MAG) {
+        const Cx<R> a00 = u.e[I][I], a01 = u.e[I][J], a10 = u.e[J][I], a11 = u.e[J][J];
+        v[0] = abs2(a00) - abs2(a01) - abs2(a10) + abs2(a11);
+        const Cx<R> c = UP ? a00 * conj(a10) - a01 * conj(a11) : conj(a00) * a01 - conj(a10) * a11;
+        v[1] = c.y;
+        v[2] = c.x;
+        v[3] = 0;
+    } else {
+        quat<R, I, J>(u, v);
+    }
+}
+
+// Reference accumulation of the eight contributions of one site in ladder
+// order; q[] receives the SU(2) element to apply (after the update rule and
+// the gauge-specific normalisation).
+template <typename R, int G, int A, bool E>
+__device__ __forceinline__ void site_update(const R (&c)[8][4], R q[4], float param, Philox &rng)
+{
+    R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (G == GF_LANDAU || G == GF_COULOMB) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (G == GF_COULOMB && m == 0) continue;
+            s0 += c[m][0]; s1 -= c[m][1]; s2 -= c[m][2]; s3 -= c[m][3];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (G == GF_COULOMB && m == 0) continue;
+            s0 += c[4 + m][0]; s1 += c[4 + m][1]; s2 += c[4 + m][2]; s3 += c[4 + m][3];
+        }
+        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = s3;
+        calc_update<R, A, E>(q, param, rng);
+    } else if (G == GF_U1XU1) {
+#pragma unroll
+        for (int m = 0; m < 4; m++) { s0 += c[m][0]; s3 -= c[m][3]; }
+#pragma unroll
+        for (int m = 0; m < 4; m++) { s0 += c[4 + m][0]; s3 += c[4 + m][3]; }
+        q[0] = s0; q[1] = 0; q[2] = 0; q[3] = s3;
+        calc_update<R, A, E>(q, param, rng);
+        q[1] = 0;
+        q[2] = 0;
+        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
+    } else {  // MAG (GaugeFixingSubgroupStep.hxx:529-575)
+#pragma unroll
+        for (int m = 0; m < 8; m++) { s0 += c[m][0]; s1 += c[m][1]; s2 += c[m][2]; }
+        s1 *= (R)2;
+        s2 *= (R)2;
+        s0 = s0 + sqrt_(s0 * s0 + s1 * s1 + s2 * s2);
+        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = 0;
+        calc_update<R, A, E>(q, param, rng);
+        q[3] = 0;
+        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
+    }
+}
+
+// ------------------------------------------------------- fused theta / gff
+// Per lane: the traceless anti-hermitian part of (U_up - U_dn) as 9 reals
+// (Im M00, Im M11, Im M22, M01, M02, M12) with M = (D - tr D/3) - h.c., and
+// Re tr U_up + Re tr U_dn -- a kernel of one parity touches every link once
+// (as the up link of a site of that parity or as the down link, i.e. the up
+// link of its neighbour), so these sum to the whole lattice's functional.
+// MAG: the three complex X_k parts and the sum of |U_jj|^2 of both links.
+constexpr int kThetaVals = 10;
+
+template <typename R, int G>
+__device__ __forceinline__ void theta_contrib(const M3<R> &up, const M3<R> &dn, bool use, R v[kThetaVals])
+{
+#pragma unroll
+    for (int k = 0; k < kThetaVals; k++) v[k] = 0;
+    if (!use) return;
+    if (G == GF_MAG) {
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = i + 1; j < 3; j++) {
+                const Cx<R> cu = up.e[i][i] * conj(up.e[j][i]) - up.e[i][j] * conj(up.e[j][j]);
+                const Cx<R> cd = conj(dn.e[i][i]) * dn.e[i][j] - conj(dn.e[j][i]) * dn.e[j][j];
+                v[k++] = cu.x + cd.x;
+                v[k++] = cu.y + cd.y;
+            }
+        v[9] = (abs2(up.e[0][0]) + abs2(up.e[1][1]) + abs2(up.e[2][2])) +
+               (abs2(dn.e[0][0]) + abs2(dn.e[1][1]) + abs2(dn.e[2][2]));
+    } else {
+        M3<R> d;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) d.e[i][j] = up.e[i][j] - dn.e[i][j];
+        const R imtr3 = (d.e[0][0].y + d.e[1][1].y + d.e[2][2].y) / (R)3;
+        v[0] = (R)2 * (d.e[0][0].y - imtr3);
+        v[1] = (R)2 * (d.e[1][1].y - imtr3);
+        v[2] = (R)2 * (d.e[2][2].y - imtr3);
+        v[3] = d.e[0][1].x - d.e[1][0].x;
+        v[4] = d.e[0][1].y + d.e[1][0].y;
+        v[5] = d.e[0][2].x - d.e[2][0].x;
+        v[6] = d.e[0][2].y + d.e[2][0].y;
+        v[7] = d.e[1][2].x - d.e[2][1].x;
+        v[8] = d.e[1][2].y + d.e[2][1].y;
+        v[9] = (up.e[0][0].x + up.e[1][1].x + up.e[2][2].x) + (dn.e[0][0].x + dn.e[1][1].x + dn.e[2][2].x);
+    }
+}
+
+template <int G> __device__ __forceinline__ void theta_site(const double s[kThetaVals], double &th, double &gf)
+{
+    if (G == GF_MAG) {
+        th = sqrt(s[0] * s[0] + s[1] * s[1]) + sqrt(s[2] * s[2] + s[3] * s[3]) + sqrt(s[4] * s[4] + s[5] * s[5]);
+    } else {
+        th = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] +
+             2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5] + s[6] * s[6] + s[7] * s[7] + s[8] * s[8]);
+    }
+    gf = s[9];
+}
+
+__device__ __forceinline__ double wave64_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------- kernel
+// THETA: 0 none, 1 theta of the even sites before the update (even kernel),
+// 2 theta of the odd sites after the update (odd kernel); each block writes its
+// (theta, gff) sums to a.partial[2*block .. 2*block+1].
+// GACC: also left-multiply the site's gauge transformation g(x) (compact
+// FP32 rows 0,1 in a.gacc[idx + V*comp]) by the same SU(2) elements.
+template <typename R, int G, int A, int S, bool EXACT, int THETA, bool GACC>
+__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
+{
+    constexpr int kQ = 8 * 4 * 64;  // one exchange buffer: [link 8][comp 4][lane 64]
+    constexpr int kT = kThetaVals * 4 * 64;
+    constexpr int kLds = (2 * kQ > kT ? 2 * kQ : kT);
+    __shared__ R lds[kLds];
+
+    if (a.state && a.state->done) return;  // converged: uniform early exit
+
+    const int mu = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
+    const bool active = p < a.g.Vh;
+    const int64_t site = (active ? p : 0) + (a.parity ? a.g.Vh : 0);
+    int c[4], par;
+    site_coords(a.g, site, c, par);
+    const bool time_link = (G == GF_COULOMB) && (mu == 0);
+    const int64_t nb = time_link ? site : backward(a.g, c, par, mu);
+    const LinkRef<R> lu = link_ref(a.U, a.g.V, S, site, mu);
+    const LinkRef<R> ld = link_ref(time_link ? a.Udw : a.U, a.g.V, S, nb, mu);
+    M3<R> up, dn;
+    if (active) {
+        load12(lu, up);
+        load12(ld, dn);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) { up.e[i][j] = cx((R)(i == j), (R)0); dn.e[i][j] = up.e[i][j]; }
+    }
+    reconstruct_e<EXACT>(up);
+    reconstruct_e<EXACT>(dn);
+
+    M3<float> gt;
+    LinkRef<float> lg;
+    const bool own_g = GACC && mu == 0 && active;
+    if (GACC) {
+        lg = link_ref(a.gacc, a.g.V, 12, site, 0);
+        if (own_g) {
+            load12(lg, gt);
+            reconstruct_e<false>(gt);
+        }
+    }
+
+    Philox rng;
+    if (A == GF_SA || A == GF_SR || A == GF_RANDOM) {
+        // key word 0 = global thread id of the reference's (mu=0, down=0) thread
+        const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
+        philox_init(rng, tid, a.seed, a.counter);
+    }
+
+    double th = 0, gf = 0;
+    auto theta_pass = [&]() {
+        R v[kThetaVals];
+        theta_contrib<R, G>(up, dn, active && !time_link, v);
+#pragma unroll
+        for (int k = 0; k < kThetaVals; k++) lds[(k * 4 + mu) * 64 + lane] = v[k];
+        __syncthreads();
+        if (mu == 0) {
+            double s[kThetaVals];
+#pragma unroll
+            for (int k = 0; k < kThetaVals; k++)
+                s[k] = ((double)lds[(k * 4 + 0) * 64 + lane] + (double)lds[(k * 4 + 1) * 64 + lane]) +
+                       ((double)lds[(k * 4 + 2) * 64 + lane] + (double)lds[(k * 4 + 3) * 64 + lane]);
+            double t, g;
+            theta_site<G>(s, t, g);
+            th = wave64_sum(active ? t : 0.0);
+            gf = wave64_sum(active ? g : 0.0);
+        }
+        __syncthreads();
+    };
+
+    if (THETA == 1) theta_pass();
+
+#define SU3GF_SUBGROUP(I, J, BUF)                                                            \
+    {                                                                                        \
+        R* const bq = lds + (BUF) * kQ;                                                      \
+        R vu[4], vd[4];                                                                      \
+        link_contrib<R, G, I, J, true>(up, vu);                                              \
+        link_contrib<R, G, I, J, false>(dn, vd);                                             \
+        _Pragma("unroll") for (int k = 0; k < 4; k++)                                        \
+        {                                                                                    \
+            bq[((0 + mu) * 4 + k) * 64 + lane] = vu[k];                                      \
+            bq[((4 + mu) * 4 + k) * 64 + lane] = vd[k];                                      \
+        }                                                                                    \
+        __syncthreads();                                                                     \
+        R cc[8][4];                                                                          \
+        _Pragma("unroll") for (int m = 0; m < 8; m++)                                        \
+            _Pragma("unroll") for (int k = 0; k < 4; k++) cc[m][k] = bq[(m * 4 + k) * 64 + lane]; \
+        R q[4];                                                                              \
+        site_update<R, G, A, EXACT>(cc, q, a.param, rng);                                    \
+        left_mult<R, I, J>(up, q);                                                           \
+        right_mult_dag<R, I, J>(dn, q);                                                      \
+        if (GACC && own_g) {                                                                 \
+            const float qf[4] = {(float)q[0], (float)q[1], (float)q[2], (float)q[3]};        \
+            left_mult<float, I, J>(gt, qf);                                                  \
+        }                                                                                    \
+    }
+    SU3GF_SUBGROUP(0, 2, 0)
+    SU3GF_SUBGROUP(1, 2, 1)
+    SU3GF_SUBGROUP(0, 1, 0)
+#undef SU3GF_SUBGROUP
+
+    if (active) {
+        store12(lu, up);
+        store12(ld, dn);
+    }
+    if (GACC && own_g) store12(lg, gt);
+
+    if (THETA == 2) {
+        __syncthreads();  // the last subgroup's readers are done with the buffers
+        theta_pass();
+    }
+    if (THETA != 0 && threadIdx.x == 0) {
+        // thread 0 (wave 0) holds the block's sums; a separate check kernel
+        // reduces the partials of both parities in a fixed order
+        a.partial[2 * blockIdx.x] = th;
+        a.partial[2 * blockIdx.x + 1] = gf;
+    }
+}
+
+}  // namespace su3gf

@@ -1,6 +1,8 @@
 """Benchmark: Landau OR gauge fixing of a 32^4 lattice to theta < 1e-12 on one
-MI355X, mixed precision (FP32 compact links until theta < 1e-6, then FP64)
-against plain FP64, reported as site-updates/s and time-to-theta.
+MI355X, mixed precision (FP32 compact links with an accumulated gauge
+transformation until the FP32 theta < 1e-12, then FP64 until the FP64 theta
+< 1e-12) against plain FP64, reported as site-updates/s and time-to-theta.
+The schedule that switches at theta < 1e-6 is reported beside it.
 
 One step = one complete gauge-fixing run of the same synthetic configuration
 (U = exp(i sum_a c_a lambda_a/2), c_a ~ N(0, 0.25^2), seed 1, then a Haar
@@ -46,8 +48,10 @@ def parse():
     ap.add_argument("--sigma", type=float, default=0.25)
     ap.add_argument("--omega", type=float, default=1.7)
     ap.add_argument("--precision", type=float, default=1e-12)
-    ap.add_argument("--switch", type=float, default=1e-6)
+    ap.add_argument("--switch", type=float, default=1e-12)
+    ap.add_argument("--switch-northstar", type=float, default=1e-6)
     ap.add_argument("--refresh", type=int, default=50)
+    ap.add_argument("--check32", type=int, default=10)
     ap.add_argument("--reproject", type=int, default=100)
     ap.add_argument("--maxiter", type=int, default=20000)
     ap.add_argument("--kernel-sweeps", type=int, default=50)
@@ -145,10 +149,14 @@ def main():
     stream = C.c_void_p()
     _lib.call("gf_stream_create", C.byref(stream))
 
+    def reset():
+        # stream-ordered copy of the resident input: part of the timed step
+        _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes), stream)
+
     def one_mixed():
-        _lib.call("gf_memcpy_d2d", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes))
+        reset()
         return gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
-                            args.refresh, args.reproject, stream=stream)
+                            args.refresh, args.check32, args.reproject, stream=stream)
 
     for w in range(args.warmup):
         r = one_mixed()
@@ -171,19 +179,27 @@ def main():
     # plain FP64 reference run (same driver, FP64 18-real layout throughout)
     fp64 = None
     if not args.no_fp64:
-        _lib.call("gf_memcpy_d2d", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes))
+        reset()
         fp64 = gf.run_fast(lat64, work, gf.LANDAU, args.omega, args.precision, args.maxiter, args.reproject,
                            stream=stream)
         print(f"bench: fp64 {fp64}", file=sys.stderr, flush=True)
+
+    # the same run with the FP32 -> FP64 switch at theta < 1e-6
+    ns = None
+    if not args.no_fp64:
+        reset()
+        ns = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch_northstar, args.precision, args.maxiter,
+                          args.refresh, args.check32, args.reproject, stream=stream)
+        print(f"bench: mixed, switch {args.switch_northstar:g}: {ns}", file=sys.stderr, flush=True)
 
     # kernel timing for the roofline (both precisions, on the launch stream)
     U32 = gf.DeviceBuffer(lat32.nbytes)
     _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(U32.ptr), 0,
               stream)
-    _lib.call("gf_memcpy_d2d", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes))
+    reset()
     k32 = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, True)
     k64 = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, False)
-    _lib.call("gf_memcpy_d2d", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes))
+    reset()
     kdrop = dropin_kernel_ms(lat64, work, stream, args.kernel_sweeps)
     # algorithmic bytes per parity launch: V/2 site-updates x 16 R sizeof(real), R = 12;
     # the FP32 kernel also reads and writes the 12-float gauge transformation of its V/2 sites
@@ -228,6 +244,7 @@ def main():
             "omega": args.omega,
             "precision": args.precision,
             "switch_precision": args.switch,
+            "check32": args.check32,
             "checkprecision": 1,
             "refresh": args.refresh,
             "reproject": args.reproject,
@@ -242,6 +259,9 @@ def main():
         "gff_fp64": None if fp64 is None else fp64.gff,
         "gff_rel_diff_mixed_vs_fp64": None if fp64 is None else abs(r.gff - fp64.gff) / abs(fp64.gff),
         "speedup_mixed_vs_fp64": None if fp64 is None else fp64.ms_total / r.ms_total,
+        "mixed_switch_1e-6": None if ns is None else {
+            "time_to_theta_s": ns.ms_total / 1e3, "sweeps": ns.sweeps, "sweeps_fp32": ns.sweeps_fp32,
+            "gff": ns.gff, "speedup_vs_fp64": fp64.ms_total / ns.ms_total},
         "site_updates_per_s_fp64": None if fp64 is None else fp64.sweeps * V / (fp64.ms_total / 1e3),
         "sweep_ms_in_run": {"fp32": t32, "fp64": t64},
         "kernel_ms": {"sweep_f32_r12_gauge": k32, "sweep_f64_pattern18": k64, "dropin_sweep_f64_pattern18": kdrop},

@@ -61,7 +61,8 @@ const char *gf_version(void);
 
 /* ------------------------------------------------- device and memory
  * Thin wrappers over the HIP runtime so that callers without a HIP binding
- * (ctypes, cgo, JNI) can own device buffers.  These replace the apps'
+ * (ctypes, cgo, JNI) can own device buffers.  The blocking copies and
+ * gf_memset return after the device has finished them.  These replace the apps'
  * cudaMalloc / cudaMemcpy / cudaFree calls
  * (gaugefixing/apps/LandauGaugeFixingSU3_4D.cu:98-116,189,280). */
 gf_status gf_device_info(int *count, char *name, size_t name_len);
@@ -69,7 +70,9 @@ gf_status gf_malloc(void **ptr, size_t bytes);
 gf_status gf_free(void *ptr);
 gf_status gf_memcpy_h2d(void *dst, const void *src, size_t bytes);
 gf_status gf_memcpy_d2h(void *dst, const void *src, size_t bytes);
-gf_status gf_memcpy_d2d(void *dst, const void *src, size_t bytes);
+gf_status gf_memcpy_d2d(void *dst, const void *src, size_t bytes);   /* returns when done */
+/* stream-ordered copy of any direction (host buffers should be pinned) */
+gf_status gf_memcpy_async(void *dst, const void *src, size_t bytes, void *stream);
 gf_status gf_memset(void *ptr, int value, size_t bytes);
 gf_status gf_synchronize(void);
 gf_status gf_stream_create(void **stream);
@@ -149,7 +152,7 @@ typedef struct {
     double theta;        /* last evaluated theta */
     double ms_total;     /* device time of the whole run (hipEvents) */
     double ms_fp32;      /* device time of the FP32 phase (mixed driver) */
-    int converged;       /* theta < precision reached */
+    int converged;       /* 1: theta < precision reached (2: a phase stagnated) */
 } gf_run_result;
 
 /* OR loop with the reference's control semantics
@@ -186,13 +189,15 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  * accumulates each site's gauge transformation g(x) in FP32; every `refresh`
  * sweeps the FP32 links are rebuilt as G U0 G^dagger from U0 and the FP64
  * projection G of g, so FP32 rounding never alters the gauge orbit.  Once
- * theta < switch_precision, U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 and
- * fused-check FP64 sweeps continue to theta < precision.  No reference
+ * theta < switch_precision (checked every check32 sweeps; the FP32 phase also
+ * ends after 50 checks in a row that do not halve its best theta), U <- G(x) U0_mu(x)
+ * G(x+mu)^dagger in FP64 and fused-check FP64 sweeps continue to
+ * theta < precision, checked every sweep.  No reference
  * counterpart (the reference's DPUPDATES is a compile-time FP32/FP64 mix,
  * gaugefixing/apps/Makefile:46-48). */
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,
-                       float precision, int maxiter, int refresh, int reproject, gf_run_result *out,
-                       void *stream);
+                       float precision, int maxiter, int refresh, int check32, int reproject,
+                       gf_run_result *out, void *stream);
 
 /* --------------------------------------------------- synthetic inputs
  * Host-side generator of the benchmark configurations (not a reference

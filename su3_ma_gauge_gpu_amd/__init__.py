@@ -371,11 +371,11 @@ class RunResult:
     theta: float
     ms_total: float
     ms_fp32: float
-    converged: bool
+    converged: int
 
 
 def _result(r: _lib.gf_run_result) -> RunResult:
-    return RunResult(r.sweeps, r.sweeps_fp32, r.checks, r.gff, r.theta, r.ms_total, r.ms_fp32, bool(r.converged))
+    return RunResult(r.sweeps, r.sweeps_fp32, r.checks, r.gff, r.theta, r.ms_total, r.ms_fp32, int(r.converged))
 
 
 def run_or(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, precision: float = 1e-7,
@@ -398,14 +398,16 @@ def run_fast(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, preci
 
 
 def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,
-              precision: float = 1e-12, maxiter: int = 20000, refresh: int = 50, reproject: int = 100,
-              stream=None) -> RunResult:
+              precision: float = 1e-12, maxiter: int = 20000, refresh: int = 50, check32: int = 10,
+              reproject: int = 100, stream=None) -> RunResult:
     """FP32 sweeps with an accumulated gauge transformation down to
-    ``switch_precision``, the transformation applied to the FP64 configuration,
-    then FP64 fused-check sweeps to ``precision``."""
+    ``switch_precision`` (checked every ``check32`` sweeps), the transformation
+    applied to the FP64 configuration, then FP64 fused-check sweeps to
+    ``precision``."""
     r = _lib.gf_run_result()
     _lib.call("gf_run_mixed", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(switch_precision),
-              C.c_float(precision), int(maxiter), int(refresh), int(reproject), C.byref(r), _ptr(stream))
+              C.c_float(precision), int(maxiter), int(refresh), int(check32), int(reproject), C.byref(r),
+              _ptr(stream))
     return _result(r)
 
 

@@ -69,6 +69,7 @@ SIGNATURES = {
     "gf_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),
     "gf_memcpy_d2h": (_i, [_vp, _vp, C.c_size_t]),
     "gf_memcpy_d2d": (_i, [_vp, _vp, C.c_size_t]),
+    "gf_memcpy_async": (_i, [_vp, _vp, C.c_size_t, _vp]),
     "gf_memset": (_i, [_vp, _i, C.c_size_t]),
     "gf_synchronize": (_i, []),
     "gf_stream_create": (_i, [C.POINTER(_vp)]),
@@ -90,7 +91,7 @@ SIGNATURES = {
     "gf_run_or": (_i, [_pl, _vp, _i, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_run_fast": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
-    "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_make_config": (_i, [_pi4, _d, _u32, _u32, C.POINTER(_d)]),
 }
 

@@ -155,12 +155,21 @@ gf_status gf_memcpy_d2h(void *dst, const void *src, size_t bytes)
 }
 gf_status gf_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
+    // a device-to-device hipMemcpy may return before the copy is done; wait for
+    // it so that work on any stream sees the copied data
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipDeviceSynchronize());
+    return GF_OK;
+}
+gf_status gf_memcpy_async(void *dst, const void *src, size_t bytes, void *stream)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream)));
     return GF_OK;
 }
 gf_status gf_memset(void *ptr, int value, size_t bytes)
 {
     HIP_TRY(hipMemset(ptr, value, bytes));
+    HIP_TRY(hipDeviceSynchronize());
     return GF_OK;
 }
 gf_status gf_synchronize(void)
@@ -502,14 +511,21 @@ double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12
 // once.  On return r->theta / r->gff describe the state after the first sweep
 // that met `tol` (the buffer then holds that state plus one even half-sweep),
 // or the final state if maxiter was reached.
+// The end state of sweep `it` is checked when (it + 1) % check_every == 0:
+// its odd kernel forms the odd-site theta, the next even kernel the even-site
+// theta before updating, and a check kernel completes the sum.  The phase ends
+// once theta < tol or, with stall_limit > 0, after stall_limit checks in a row
+// that do not halve the smallest theta seen (r->converged = 2).
 template <typename R, typename After>
 gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
-                      FusedCtx &fc, hipStream_t s, After after_sweep, int *sweeps_done, gf_run_result *r)
+                      int check_every, int stall_limit, FusedCtx &fc, hipStream_t s, After after_sweep,
+                      int *sweeps_done, gf_run_result *r)
 {
     int64_t V = 1;
     for (int i = 0; i < 4; i++) V *= lat->dims[i];
     const Geom g = make_geom(lat->dims, V);
-    hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, s);
+    if (check_every < 1) check_every = 1;
+    hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit, s);
     if (e != hipSuccess) return hip_fail(e, "state init");
     SweepArgs<R> a{};
     a.U = U;
@@ -524,18 +540,20 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
     while (it < maxiter && !done) {
         const int n = std::min(kChunk, maxiter - it);
         for (int k = 0; k < n; k++, it++) {
+            const bool check_prev = it > 0 && it % check_every == 0;   // end of sweep it-1
+            const bool check_this = (it + 1) % check_every == 0;       // end of sweep it
             a.sweep = it;
             a.parity = 0;
             a.partial = fc.even();
-            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, 1, gacc != nullptr, a, s);
+            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, check_prev ? 1 : 0, gacc != nullptr, a, s);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
-            if (it > 0) {
+            if (check_prev) {
                 e = launch_fused_check(fc.st, fc.even(), fc.odd(), fc.nblocks, it, s);
                 if (e != hipSuccess) return hip_fail(e, "check kernel launch");
             }
             a.parity = 1;
             a.partial = fc.odd();
-            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, 2, gacc != nullptr, a, s);
+            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, check_this ? 2 : 0, gacc != nullptr, a, s);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
             GF_TRY(after_sweep(it));
         }
@@ -547,7 +565,7 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         *sweeps_done = fc.host->converged_sweep + 1;
         r->theta = fc.host->theta;
         r->gff = fc.host->gff;
-        r->converged = 1;
+        r->converged = fc.host->done;
     } else {
         *sweeps_done = it;
         r->converged = 0;
@@ -591,10 +609,10 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     };
     int done = 0;
     gf_status rc = lat->dtype == GF_F64
-                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, fc, s, after,
-                                             &done, out)
-                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, fc, s, after,
-                                            &done, out);
+                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, fc, s,
+                                             after, &done, out)
+                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, fc, s,
+                                            after, &done, out);
     GF_TRY(rc);
     out->sweeps = done;
     GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
@@ -629,7 +647,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
-    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, s));
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -669,8 +687,9 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
 }
 
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision, float precision,
-                       int maxiter, int refresh, int reproject, gf_run_result *out, void *stream)
+                       int maxiter, int refresh, int check32, int reproject, gf_run_result *out, void *stream)
 {
+    constexpr int kStall32 = 50;  // FP32 checks without halving theta before switching anyway
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
@@ -722,8 +741,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         return GF_OK;
     };
     int done32 = 0;
-    GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, fc, s, after32, &done32,
-                              out));
+    GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32, kStall32, fc,
+                              s, after32, &done32, out));
     out->sweeps_fp32 = done32;
     // switch: apply the accumulated transformation to the FP64 configuration
     HIP_TRY(launch_project_gauge(g, V, s));
@@ -736,7 +755,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     int done64 = 0;
     out->converged = 0;
     const int left = std::max(0, maxiter - done32);
-    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, fc, s, after64, &done64, out));
+    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, 1, 0, fc, s, after64, &done64,
+                               out));
     out->sweeps = done32 + done64;
     GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
     HIP_TRY(hipEventRecord(ev.b, s));

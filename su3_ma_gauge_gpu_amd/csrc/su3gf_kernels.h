@@ -11,11 +11,13 @@ constexpr int kBlock = 256;  // 4 waves of 64 lanes: one wave per direction, 64 
 // Device-resident state of a fused-theta driver run.  Written by the check
 // kernel that follows each even sweep kernel; read by the host after a chunk.
 struct DevState {
-    int done;             // theta < tol reached
-    int converged_sweep;  // index of the first sweep whose end state met tol
+    int done;             // 1: theta < tol reached; 2: theta stagnated
+    int converged_sweep;  // index of the sweep whose end state met tol (or stagnated)
     int checked_sweep;    // index of the sweep whose end state `theta` describes
     int checks;           // number of completed checks
-    int pad0_, pad1_;
+    int stall;            // consecutive checks without halving the best theta
+    int stall_limit;      // 0: no stagnation exit
+    double best;          // smallest theta seen
     double tol;           // theta threshold
     double theta_scale;   // 1 / (Nc * V)
     double gff_scale;     // 1 / (Nc * Ndim * V)
@@ -73,7 +75,8 @@ hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s);
 // U0 FP64 in layout s0; out of type `to` in layout so (may alias U0 if same type/layout)
 hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *out, int to, int so, const Geom &geo,
                               hipStream_t s);
-hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, hipStream_t s);
+hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
+                             hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep
 // `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update)
 hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int sweep,

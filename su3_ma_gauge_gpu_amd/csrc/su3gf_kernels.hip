@@ -206,15 +206,16 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
 }
 
 // ------------------------------------------------------------ driver state
-__global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale)
+__global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     st->done = 0;
     st->converged_sweep = -1;
     st->checked_sweep = -1;
     st->checks = 0;
-    st->pad0_ = 0;
-    st->pad1_ = 0;
+    st->stall = 0;
+    st->stall_limit = stall_limit;
+    st->best = 1e300;
     st->tol = tol;
     st->theta_scale = theta_scale;
     st->gff_scale = gff_scale;
@@ -222,9 +223,10 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
     st->gff = 0;
 }
 
-hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, hipStream_t s)
+hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
+                             hipStream_t s)
 {
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale);
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale, stall_limit);
     return hipGetLastError();
 }
 
@@ -260,8 +262,17 @@ __global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double
         st->gff = sb[0] * st->gff_scale;
         st->checked_sweep = sweep - 1;
         st->checks++;
+        if (theta < 0.5 * st->best) {
+            st->best = theta;
+            st->stall = 0;
+        } else {
+            st->stall++;
+        }
         if (theta < st->tol) {
             st->done = 1;
+            st->converged_sweep = sweep - 1;
+        } else if (st->stall_limit > 0 && st->stall >= st->stall_limit) {
+            st->done = 2;
             st->converged_sweep = sweep - 1;
         }
     }

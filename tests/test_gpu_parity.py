@@ -264,10 +264,11 @@ def test_run_mixed_matches_fp64_oracle(gf, gpu, oracle, dims, sigma):
     maximum)."""
     U = _config(gf, dims, sigma, 1, 2)
     d = gf.DeviceBuffer.from_numpy(U)
-    res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, 1e-6, 1e-12, 20000, 50, 100)
+    res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=1e-6, precision=1e-12,
+                       maxiter=20000, refresh=50, check32=10, reproject=100)
     Uo = U.copy()
     n, g, t = oracle.run_or(LANDAU, dims, Uo, 1.7, 1e-12, 20000, 1, 100)
-    assert res.converged and res.theta < 1e-12
+    assert res.converged == 1 and res.theta < 1e-12
     assert 0 < res.sweeps_fp32 < res.sweeps
     assert res.gff == pytest.approx(g, rel=1e-10)
     # the result is a gauge transform of the input: the plaquette is unchanged
@@ -296,3 +297,18 @@ def _plaquette(M, dims):
             P = M[mu, here] @ M[nu, xm] @ np.conj(np.swapaxes(M[mu, xn], -1, -2)) @ np.conj(np.swapaxes(M[nu, here], -1, -2))
             tot += np.trace(P, axis1=-2, axis2=-1).real.sum()
     return tot / (6 * V * 3)
+
+
+def test_run_mixed_fp32_stagnation(gf, gpu, oracle):
+    """With a switch threshold FP32 cannot reach, the FP32 phase ends on
+    stagnation and the FP64 phase still converges to the oracle's maximum."""
+    dims = (8, 8, 8, 8)
+    U = _config(gf, dims, 0.3, 1, 2)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=1e-30, precision=1e-12,
+                       maxiter=20000, refresh=50, check32=5, reproject=100)
+    Uo = U.copy()
+    n, g, t = oracle.run_or(LANDAU, dims, Uo, 1.7, 1e-12, 20000, 1, 100)
+    assert res.converged == 1 and res.theta < 1e-12
+    assert 0 < res.sweeps_fp32 < res.sweeps
+    assert res.gff == pytest.approx(g, rel=1e-10)

@@ -1,0 +1,142 @@
+"""The C ABI boundary without a GPU: the library loads, exports every function
+declared in include/su3gf.h, rejects bad arguments on the host with the
+documented status codes, and the host-side pieces (configuration generator,
+the division-by-constant used for site decoding) are correct."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    text = (ROOT / "include" / "su3gf.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported(gf):
+    lib = gf._lib.load()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in su3gf.h but not exported"
+    # and the Python binding knows every one of them
+    assert set(names) == set(gf._lib.SIGNATURES)
+
+
+def test_version(gf):
+    assert b"gfx950" in gf._lib.load().gf_version()
+
+
+def _lat(gf, dims, dtype=1, layout=18):
+    return gf._lib.gf_lattice((C.c_int * 4)(*dims), dtype, layout)
+
+
+@pytest.mark.parametrize("dims,status", [((4, 4, 4, 3), 1), ((4, 5, 4, 4), 1), ((3, 4, 4, 4), 1), ((0, 4, 4, 4), 1)])
+def test_sweep_rejects_bad_shapes(gf, dims, status):
+    lib = gf._lib.load()
+    lat = _lat(gf, dims)
+    rc = lib.gf_sweep(C.byref(lat), C.c_void_p(0x1000), None, 0, 0, 0, C.c_float(1.7), 0, 0, 32, None)
+    assert rc == status
+    assert "odd" in gf._lib.last_error() or "positive" in gf._lib.last_error()
+
+
+def test_sweep_rejects_bad_arguments(gf):
+    lib = gf._lib.load()
+    lat = _lat(gf, (4, 4, 4, 4))
+    good = C.c_void_p(0x1000)
+    assert lib.gf_sweep(C.byref(lat), None, None, 0, 0, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+    assert lib.gf_sweep(C.byref(lat), good, None, 2, 0, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+    assert lib.gf_sweep(C.byref(lat), good, None, 0, 7, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+    assert lib.gf_sweep(C.byref(lat), good, None, 0, 0, 9, C.c_float(1.7), 0, 0, 32, None) == 2
+    # Coulomb needs a timeslice and the slice t-1 buffer
+    assert lib.gf_sweep(C.byref(lat), good, good, 0, 1, 0, C.c_float(1.7), 0, 0, 32, None) == 1
+    ts = _lat(gf, (1, 4, 4, 4))
+    assert lib.gf_sweep(C.byref(ts), good, None, 0, 1, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+    bad_dtype = _lat(gf, (4, 4, 4, 4), dtype=5)
+    assert lib.gf_sweep(C.byref(bad_dtype), good, None, 0, 0, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+    bad_layout = _lat(gf, (4, 4, 4, 4), layout=16)
+    assert lib.gf_sweep(C.byref(bad_layout), good, None, 0, 0, 0, C.c_float(1.7), 0, 0, 32, None) == 2
+
+
+def test_stats_rejects_bad_arguments(gf):
+    lib = gf._lib.load()
+    h = C.c_void_p()
+    assert lib.gf_stats_create((C.c_int * 4)(4, 4, 4, 4), 9, 1, C.byref(h)) == 2
+    assert lib.gf_stats_create((C.c_int * 4)(4, 4, 4, 4), 0, 5, C.byref(h)) == 2
+    assert lib.gf_stats_create((C.c_int * 4)(4, 4, 3, 4), 0, 1, C.byref(h)) == 1
+
+
+def test_lattice_bytes(gf):
+    lib = gf._lib.load()
+    assert lib.gf_lattice_bytes(C.byref(_lat(gf, (4, 4, 4, 4)))) == 256 * 4 * 18 * 8
+    assert lib.gf_lattice_bytes(C.byref(_lat(gf, (4, 4, 4, 4), 0, 12))) == 256 * 4 * 12 * 4
+    assert lib.gf_lattice_bytes(C.byref(_lat(gf, (4, 4, 4, 3)))) == 0
+
+
+def test_python_mirror_checks_launch_geometry(gf):
+    """The reference's launch (V/2/NSB blocks of 8*NSB threads) is validated
+    before any device call (LandauGaugeFixingSU3_4D.cu:124-125)."""
+    lat = gf.Lattice((4, 4, 4, 4))
+    K = gf.LandauKernelsSU3(lat)
+    with pytest.raises(gf.GFError):
+        K.orStep(3, 256, 0x1000, None, 0, 1.7)
+    with pytest.raises(gf.GFError):
+        K.orStep(4, 250, 0x1000, None, 0, 1.7)
+    with pytest.raises(gf.GFError):
+        gf.CommonKernelsSU3(lat).projectSU3(4, 32, 0x1000)
+
+
+def test_make_config_properties(gf):
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = gf.make_config(dims, 0.3, 1, 2)
+    M = U.reshape(4, 3, 3, 2, V)
+    M = np.moveaxis(M[..., 0, :] + 1j * M[..., 1, :], -1, 1)
+    assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-14
+    assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
+    assert np.array_equal(U, gf.make_config(dims, 0.3, 1, 2))
+    assert not np.array_equal(U, gf.make_config(dims, 0.3, 1, 3))
+    assert not np.array_equal(U, gf.make_config(dims, 0.3, 2, 2))
+    # sigma = 0 without a transformation: unit links
+    C0 = gf.make_config(dims, 0.0, 1, 0).reshape(4, 3, 3, 2, V)
+    assert np.array_equal(C0[:, :, :, 0, :], np.broadcast_to(np.eye(3)[None, :, :, None], (4, 3, 3, V)))
+    with pytest.raises(gf.GFError):
+        gf.make_config((4, 4, 4, 3), 0.3, 1, 2)
+
+
+def test_make_config_link_distribution(gf):
+    """U = exp(i H) with H = sum_a c_a lambda_a/2: Re tr U / 3 averages to the
+    Gaussian expectation, close to 1 - sigma^2 * 4/3 * 1/2 for small sigma."""
+    dims = (4, 4, 4, 8)
+    sigma = 0.1
+    U = gf.make_config(dims, sigma, 3, 0)
+    V = int(np.prod(dims))
+    A = U.reshape(4, 3, 3, 2, V)
+    retr = (A[:, 0, 0, 0] + A[:, 1, 1, 0] + A[:, 2, 2, 0]).mean() / 3
+    # <Re tr exp(iH)>/3 = 1 - <tr H^2>/6 + O(sigma^4), <tr H^2> = 8 sigma^2 / 2
+    assert retr == pytest.approx(1 - 8 * sigma**2 / 12, abs=2e-3)
+
+
+def _fastdiv(d):
+    s = 0
+    while (1 << s) < d:
+        s += 1
+    m = ((1 << 32) * ((1 << s) - d)) // d + 1
+    return m & 0xFFFFFFFF, s
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 30, 32, 48, 64, 96, 100, 128, 1000, 4096])
+def test_fastdiv_formula(d):
+    """q = (umulhi(n, m) + n) >> s, as the kernels decode site indices, for
+    n < 2^31 (the volume limit checked by the library)."""
+    m, s = _fastdiv(d)
+    rng = np.random.default_rng(d)
+    ns = np.concatenate([np.arange(0, 5000), rng.integers(0, 2**31, 20000), [2**31 - 1, 2**31 - 2]]).astype(np.uint64)
+    hi = (ns * np.uint64(m)) >> np.uint64(32)
+    q = (hi + ns) >> np.uint64(s)
+    assert np.array_equal(q, ns // np.uint64(d))

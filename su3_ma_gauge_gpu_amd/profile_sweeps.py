@@ -1,0 +1,58 @@
+"""Run the dominant kernels of the 32^4 benchmark a fixed number of times,
+for kernel traces and hardware-counter runs:
+
+* one device-to-device copy of the FP64 18-real configuration (a plain copy
+  of known size, 4 * V * 18 * 8 bytes read and written), three times;
+* `--sweeps` fused-check FP32 sweeps with gauge accumulation on the compact
+  FP32 layout (the mixed driver's FP32 kernel);
+* `--sweeps` fused-check FP64 sweeps on the 18-real layout (the FP64 kernel);
+* `--sweeps` drop-in (reference-order) FP64 sweeps through gf_sweep.
+
+Usage: python su3_ma_gauge_gpu_amd/profile_sweeps.py [--dims 32 32 32 32] [--sweeps 10]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import sys
+from pathlib import Path
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+
+import su3_ma_gauge_gpu_amd as gf  # noqa: E402
+from su3_ma_gauge_gpu_amd import _lib  # noqa: E402
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dims", type=int, nargs=4, default=[32, 32, 32, 32])
+    ap.add_argument("--sweeps", type=int, default=10)
+    args = ap.parse_args()
+    dims = tuple(args.dims)
+    lat64 = gf.Lattice(dims, gf.F64, gf.PATTERN18)
+    lat32 = gf.Lattice(dims, gf.F32, gf.R12)
+    U0 = gf.make_config(dims, 0.25, 1, 2)
+    src = gf.DeviceBuffer.from_numpy(U0)
+    work = gf.DeviceBuffer(lat64.nbytes)
+    for _ in range(3):
+        work.copy_from(src)
+    print(f"copy: {lat64.nbytes} bytes x 3", flush=True)
+    u32 = gf.DeviceBuffer(lat32.nbytes)
+    _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(u32.ptr), 0,
+              None)
+    ms = C.c_float()
+    _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, 1, C.byref(ms), None)
+    print(f"fp32 fused sweep with gauge accumulation: {ms.value:.4f} ms per kernel", flush=True)
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 0, C.byref(ms),
+              None)
+    print(f"fp64 fused sweep: {ms.value:.4f} ms per kernel", flush=True)
+    K = gf.LandauKernelsSU3(lat64)
+    for _ in range(args.sweeps):
+        for parity in (0, 1):
+            K.orStep(lat64.V // 2 // 32, 256, work, None, parity, 1.7)
+    gf.synchronize()
+    print("drop-in sweeps done", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,6 +26,7 @@ HIP_SOURCES = [
     ("su3gf_exact.hip", ["-ffp-contract=off"]),
     ("su3gf_fast.hip", []),
     ("su3gf_kernels.hip", []),
+    ("su3gf_gauge.hip", []),
     ("su3gf_api.hip", []),
 ]
 HOST_SOURCES = ["su3gf_config.cpp"]
@@ -42,6 +43,17 @@ HIPCC_FLAGS = [
     f"-I{INCLUDE}",
 ]
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", f"-I{INCLUDE}"]
+
+# The application on the drop-in headers (include/culgt): host code only, so
+# g++ with the HIP runtime headers; the lattice extents are compile-time
+# constants as in the reference, one binary per (extent, precision).
+APPS = PKG / "apps"
+APP_VARIANTS = [
+    ("gaugefix_x8t8_f64", ["-D_X_=8", "-D_T_=8", "-DDOUBLEPRECISION"]),
+    ("gaugefix_x8t8_f32", ["-D_X_=8", "-D_T_=8"]),
+    ("gaugefix_x32t32_f64", ["-D_X_=32", "-D_T_=32", "-DDOUBLEPRECISION"]),
+]
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 
 def _hipcc() -> str:
@@ -94,7 +106,28 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lgomp"], verbose)
     os.replace(tmp, LIB)
+    build_apps(verbose)
     return LIB
+
+
+def build_apps(verbose: bool = False) -> list[Path]:
+    """Link the gauge-fixing application variants against ``libsu3gf.so``
+    into ``apps/bin/``."""
+    out = APPS / "bin"
+    out.mkdir(exist_ok=True)
+    jobs, paths = [], []
+    for name, defs in APP_VARIANTS:
+        exe = out / name
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", f"-I{INCLUDE}",
+               *defs, str(APPS / "gaugefix.cpp"), "-o", str(exe), f"-L{PKG}", "-lsu3gf", f"-L{ROCM / 'lib'}",
+               "-lamdhip64", "-Wl,-rpath,$ORIGIN/../..", f"-Wl,-rpath,{ROCM / 'lib'}"]
+        jobs.append(_start(cmd, verbose))
+        paths.append(exe)
+    failed = [out for job in jobs for out in [job.communicate()[0]] if job.returncode != 0]
+    if failed:
+        sys.stderr.write("\n".join(failed))
+        raise RuntimeError("application build failed")
+    return paths
 
 
 if __name__ == "__main__":

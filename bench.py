@@ -1,8 +1,10 @@
 """Benchmark: Landau OR gauge fixing of a 32^4 lattice to theta < 1e-12 on one
-MI355X, mixed precision (FP32 compact links with an accumulated gauge
-transformation until the FP32 theta < 1e-12, then FP64 until the FP64 theta
-< 1e-12) against plain FP64, reported as site-updates/s and time-to-theta.
-The schedule that switches at theta < 1e-6 is reported beside it.
+MI355X, mixed precision (an FP32 phase that sweeps the gauge transformation
+g(x) on a read-only FP32 copy of the links, in FP64 arithmetic, until its
+theta < 1e-12; then g applied to the FP64 links and FP64 sweeps until the
+FP64 theta < 1e-12) against plain FP64, reported as site-updates/s and
+time-to-theta.  The other single-precision modes and the schedule that
+switches at theta < 1e-6 are reported beside it.
 
 One step = one complete gauge-fixing run of the same synthetic configuration
 (U = exp(i sum_a c_a lambda_a/2), c_a ~ N(0, 0.25^2), seed 1, then a Haar
@@ -37,6 +39,7 @@ import su3_ma_gauge_gpu_amd as gf  # noqa: E402
 from su3_ma_gauge_gpu_amd import _lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MODE_NAMES = {0: "links_f32+g", 1: "gauge_f32", 2: "gauge_f32_f64arith"}
 
 
 def parse():
@@ -58,6 +61,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--mode", type=int, default=2, help="0 links+g, 1 g (FP32 arithmetic), 2 g (FP64 arithmetic)")
+    ap.add_argument("--copies", type=int, default=4)
+    ap.add_argument("--sa-steps", type=int, default=3000)
+    ap.add_argument("--sa-mode", type=int, default=1, help="0 FP64 link sweeps, 1 FP32 gauge-transformation sweeps")
+    ap.add_argument("--mag-precision", type=float, default=1e-12)
+    ap.add_argument("--workload", default="landau",
+                    choices=["landau", "coulomb", "gribov", "mag"],
+                    help="landau: the 32^4 headline run (default); coulomb: 32^3x64 all slices at once; "
+                         "gribov: SA + OR gauge copies of 16^4..32^4; mag: 48^4 MAG then U1xU1")
     return ap.parse_args()
 
 
@@ -77,12 +89,14 @@ class Events:
         return float(ms.value)
 
 
-def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, with_gauge: bool) -> float:
+def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, variant: int) -> float:
     """Average duration of one fused-check parity sweep kernel (the kernel the
     drivers run), from HIP events around nsweeps back-to-back sweeps
-    (2 * nsweeps launches) on `stream`."""
+    (2 * nsweeps launches) on `stream`.  variant: 0 link sweep in the
+    precision of `lat`, 1 FP32 links + g, 2 / 3 gauge-transformation sweep in
+    FP32 / FP64 arithmetic."""
     ms = C.c_float()
-    _lib.call("gf_time_sweeps", C.byref(lat.struct), C.c_void_p(U.ptr), gf.LANDAU, int(nsweeps), int(with_gauge),
+    _lib.call("gf_time_sweeps", C.byref(lat.struct), C.c_void_p(U.ptr), gf.LANDAU, int(nsweeps), int(variant),
               C.byref(ms), stream)
     return float(ms.value)
 
@@ -132,12 +146,173 @@ def cpu_baseline(dims, U_host: np.ndarray, seconds: float) -> dict:
     }
 
 
+def _stream():
+    stream = C.c_void_p()
+    _lib.call("gf_stream_create", C.byref(stream))
+    return stream
+
+
+def _base_line(metric, value, unit, ms_per_step, steps, warmup, config, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic", "config": config}
+
+
+def slice_quality(lat: gf.Lattice, U: gf.DeviceBuffer, stream):
+    """Per-timeslice Coulomb theta and gff of a 4D lattice (prefactors of
+    CoulombKernelsSU3.hxx:71-78 per slice): per-site values from
+    gf_quality_per_site, summed per slice on the host."""
+    V = lat.V
+    dG = gf.DeviceBuffer(8 * V)
+    dA = gf.DeviceBuffer(8 * V)
+    _lib.call("gf_quality_per_site", C.byref(lat.struct), gf.COULOMB, C.c_void_p(U.ptr), C.c_void_p(dG.ptr),
+              C.c_void_p(dA.ptr), stream)
+    _lib.call("gf_stream_synchronize", stream)
+    g = dG.download(np.float64, V).reshape(2, lat.dims[0], -1).sum(axis=(0, 2))
+    a = dA.download(np.float64, V).reshape(2, lat.dims[0], -1).sum(axis=(0, 2))
+    V3 = V // lat.dims[0]
+    return a / (3 * V3), g / (9 * V3)
+
+
+def workload_coulomb(args):
+    """Coulomb gauge of a 32^3 x 64 lattice (sigma 0.3, seed 1, Haar seed 2),
+    every timeslice at once: mixed (FP32 gauge-transformation sweeps, then
+    FP64) against all-FP64, each slice to theta < 1e-12."""
+    dims = (64, 32, 32, 32)
+    V = int(np.prod(dims))
+    U0 = gf.make_config(dims, 0.3, 1, 2)
+    lat = gf.Lattice(dims)
+    src = gf.DeviceBuffer.from_numpy(U0)
+    work = gf.DeviceBuffer(lat.nbytes)
+    stream = _stream()
+
+    def reset():
+        _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
+
+    def mixed():
+        reset()
+        return gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.switch, args.precision, args.maxiter,
+                            mode=args.mode, check32=args.check32, stream=stream)
+
+    for _ in range(args.warmup):
+        mixed()
+    rs = [mixed() for _ in range(args.steps)]
+    r = rs[-1]
+    th_m, g_m = slice_quality(lat, work, stream)
+    reset()
+    f = gf.run_fast(lat, work, gf.COULOMB, args.omega, args.precision, args.maxiter, args.reproject, stream=stream)
+    th_f, g_f = slice_quality(lat, work, stream)
+    ms = sum(x.ms_total for x in rs) / len(rs)
+    line = _base_line("Coulomb OR site-updates/s + time to theta<1e-12 per slice at 32^3x64: mixed vs FP64",
+                      r.sweeps * V / (ms / 1e3), "site-updates/s", ms, args.steps, args.warmup,
+                      {"workload": "coulomb-or-32x32x32-x64-slices-mixed", "dims": list(dims), "sigma": 0.3,
+                       "seed": 1, "haar_seed": 2, "omega": args.omega, "precision": args.precision,
+                       "switch_precision": args.switch, "mode": MODE_NAMES[args.mode]})
+    line.update({
+        "time_to_theta_mixed_s": ms / 1e3, "sweeps_mixed": r.sweeps, "sweeps_fp32": r.sweeps_fp32,
+        "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps,
+        "speedup_mixed_vs_fp64": f.ms_total / ms,
+        "max_slice_theta_mixed": float(th_m.max()), "max_slice_theta_fp64": float(th_f.max()),
+        "max_slice_gff_rel_diff": float(np.max(np.abs(g_m - g_f) / np.abs(g_f))),
+        "gff_mixed": r.gff, "gff_fp64": f.gff,
+    })
+    print(json.dumps(line), flush=True)
+
+
+def workload_mag(args):
+    """MAG of a 48^4 lattice (sigma 0.35, seed 1, Haar seed 2): mixed (FP32
+    links with g, switch at theta < 1e-6) against all-FP64 to theta <
+    precision, then 1000 U1xU1 OR sweeps on the MAG result."""
+    dims = (48, 48, 48, 48)
+    V = int(np.prod(dims))
+    U0 = gf.make_config(dims, 0.35, 1, 2)
+    lat = gf.Lattice(dims)
+    src = gf.DeviceBuffer.from_numpy(U0)
+    work = gf.DeviceBuffer(lat.nbytes)
+    stream = _stream()
+    prec = args.mag_precision
+
+    def reset():
+        _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
+
+    reset()
+    m = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, prec, args.maxiter, mode=gf.MIXED_LINKS,
+                     refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
+    print(f"bench: mag mixed {m}", file=sys.stderr, flush=True)
+    u = gf.run_fast(lat, work, gf.U1XU1, args.omega, 1e-30, 1000, args.reproject, stream=stream)
+    print(f"bench: u1xu1 {u}", file=sys.stderr, flush=True)
+    reset()
+    f = gf.run_fast(lat, work, gf.MAG, args.omega, prec, args.maxiter, args.reproject, stream=stream)
+    print(f"bench: mag fp64 {f}", file=sys.stderr, flush=True)
+    line = _base_line(f"MAG OR site-updates/s + time to theta<{prec:g} at 48^4: mixed vs FP64, then U1xU1",
+                      m.sweeps * V / (m.ms_total / 1e3), "site-updates/s", m.ms_total, 1, 0,
+                      {"workload": "mag-or-48x48x48x48-mixed+u1xu1", "dims": list(dims), "sigma": 0.35, "seed": 1,
+                       "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
+                       "mode": MODE_NAMES[0]})
+    line.update({
+        "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
+        "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,
+        "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
+        "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
+        "speedup_mixed_vs_fp64": f.ms_total / m.ms_total,
+        "u1xu1_1000_sweeps_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
+    })
+    print(json.dumps(line), flush=True)
+
+
+def workload_gribov(args):
+    """Gauge copies: for each of 16^4, 20^4, 24^4, 32^4 (sigma 0.3, seed 1),
+    four copies with Haar seeds 10.. (one per copy); each copy is annealed
+    (T 1.45 -> 0.01 in args.sa_steps steps, 3 microcanonical steps per
+    heatbath step) and then OR-fixed to theta < 1e-12 (mixed precision); the
+    best functional per size is kept."""
+    stream = _stream()
+    sizes = [16, 20, 24, 32]
+    total_ms, total_su, per_size = 0.0, 0, {}
+    ev = Events()
+    haar = 10
+    ctr = 0
+    for L in sizes:
+        dims = (L, L, L, L)
+        V = L ** 4
+        lat = gf.Lattice(dims)
+        best = None
+        copies = []
+        for c in range(args.copies):
+            U0 = gf.make_config(dims, 0.3, 1, haar)
+            haar += 1
+            d = gf.DeviceBuffer.from_numpy(U0)
+            _lib.call("gf_synchronize")
+            ev.start(stream)
+            ctr, ntemps = gf.run_sa(lat, d, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject, 1, ctr,
+                                    mode=args.sa_mode, stream=stream)
+            r = gf.run_mixed(lat, d, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
+                             mode=args.mode, check32=args.check32, stream=stream)
+            ms = ev.stop_ms(stream)
+            total_ms += ms
+            total_su += (ntemps * (1 + 3) + r.sweeps) * V
+            copies.append({"ms": ms, "or_sweeps": r.sweeps, "gff": r.gff, "theta": r.theta})
+            best = r.gff if best is None else max(best, r.gff)
+            print(f"bench: {L}^4 copy {c}: {ms:.0f} ms, {r}", file=sys.stderr, flush=True)
+        per_size[f"{L}^4"] = {"best_gff": best, "copies": copies}
+    line = _base_line("Gribov copies: site-updates/s over SA + OR of 4 copies each of 16^4..32^4",
+                      total_su / (total_ms / 1e3), "site-updates/s", total_ms, 1, 0,
+                      {"workload": "gribov-copies-16-20-24-32-sa-or", "sizes": sizes, "copies": args.copies,
+                       "sigma": 0.3, "seed": 1, "haar_seeds": [10, haar - 1], "sa": [1.45, 0.01, args.sa_steps, 3],
+                       "precision": args.precision, "mode": MODE_NAMES[args.mode]})
+    line.update({"total_s": total_ms / 1e3, "per_size": per_size})
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build is single-GPU: "
               "see DESIGN.md for how the sweep would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "landau":
+        {"coulomb": workload_coulomb, "mag": workload_mag, "gribov": workload_gribov}[args.workload](args)
+        return
     dims = tuple(args.dims)
     V = int(np.prod(dims))
     print(f"bench: generating {dims} config", file=sys.stderr, flush=True)
@@ -153,10 +328,13 @@ def main():
         # stream-ordered copy of the resident input: part of the timed step
         _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes), stream)
 
-    def one_mixed():
+    def mixed(mode, switch):
         reset()
-        return gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
-                            args.refresh, args.check32, args.reproject, stream=stream)
+        return gf.run_mixed(lat64, work, gf.LANDAU, args.omega, switch, args.precision, args.maxiter, mode=mode,
+                            refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
+
+    def one_mixed():
+        return mixed(args.mode, args.switch)
 
     for w in range(args.warmup):
         r = one_mixed()
@@ -184,32 +362,53 @@ def main():
                            stream=stream)
         print(f"bench: fp64 {fp64}", file=sys.stderr, flush=True)
 
-    # the same run with the FP32 -> FP64 switch at theta < 1e-6
+    # the same run with the FP32 -> FP64 switch at theta < 1e-6, and the other
+    # single-precision modes
     ns = None
+    modes = {}
     if not args.no_fp64:
-        reset()
-        ns = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch_northstar, args.precision, args.maxiter,
-                          args.refresh, args.check32, args.reproject, stream=stream)
+        ns = mixed(args.mode, args.switch_northstar)
         print(f"bench: mixed, switch {args.switch_northstar:g}: {ns}", file=sys.stderr, flush=True)
+        for m in (0, 1, 2):
+            rm = r if m == args.mode else mixed(m, args.switch)
+            modes[MODE_NAMES[m]] = {"time_to_theta_s": rm.ms_total / 1e3, "sweeps": rm.sweeps,
+                                    "sweeps_fp32": rm.sweeps_fp32, "gff": rm.gff,
+                                    "speedup_vs_fp64": None if fp64 is None else fp64.ms_total / rm.ms_total}
+            print(f"bench: mode {MODE_NAMES[m]}: {rm}", file=sys.stderr, flush=True)
 
-    # kernel timing for the roofline (both precisions, on the launch stream)
+    # kernel timing for the roofline (on the launch stream)
     U32 = gf.DeviceBuffer(lat32.nbytes)
-    _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(U32.ptr), 0,
-              stream)
+
+    def fresh32():
+        _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct),
+                  C.c_void_p(U32.ptr), 0, stream)
+
+    kms = {}
+    for var, name in ((1, "sweep_f32_r12_gauge"), (2, "gauge_sweep_f32"), (3, "gauge_sweep_f64arith")):
+        fresh32()
+        kms[name] = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var)
     reset()
-    k32 = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, True)
-    k64 = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, False)
+    kms["sweep_f64_pattern18"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 0)
     reset()
-    kdrop = dropin_kernel_ms(lat64, work, stream, args.kernel_sweeps)
-    # algorithmic bytes per parity launch: V/2 site-updates x 16 R sizeof(real), R = 12;
-    # the FP32 kernel also reads and writes the 12-float gauge transformation of its V/2 sites
-    bytes32 = (V // 2) * (16 * 12 * 4 + 2 * 12 * 4)
-    bytes64 = (V // 2) * 16 * 12 * 8
+    kms["dropin_sweep_f64_pattern18"] = dropin_kernel_ms(lat64, work, stream, args.kernel_sweeps)
+    # algorithmic bytes per parity launch (V/2 site-updates):
+    #   link sweeps: 16 R sizeof(real), R = 12 (rows 0,1 of every link read and written once);
+    #   FP32 links + g: also the site's 12-float g read and written (+96 B);
+    #   gauge-transformation sweep: U0 read once (8 x 48 B), g of the other parity read once
+    #   (48 B), the site's g read and written (96 B): 528 B
+    kbytes = {
+        "sweep_f32_r12_gauge": (V // 2) * (16 * 12 * 4 + 2 * 12 * 4),
+        "gauge_sweep_f32": (V // 2) * 528,
+        "gauge_sweep_f64arith": (V // 2) * 528,
+        "sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
+        "dropin_sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
+    }
     t32 = r.ms_fp32 / max(r.sweeps_fp32, 1)
     t64 = (r.ms_total - r.ms_fp32) / max(r.sweeps - r.sweeps_fp32, 1)
-    dom32 = r.sweeps_fp32 * k32 >= (r.sweeps - r.sweeps_fp32) * k64
-    kname, kms, kbytes = ("sweep_f32_r12_gauge", k32, bytes32) if dom32 else ("sweep_f64_pattern18", k64, bytes64)
-    achieved = kbytes / (kms / 1e3) / 1e9
+    k32name = {0: "sweep_f32_r12_gauge", 1: "gauge_sweep_f32", 2: "gauge_sweep_f64arith"}[args.mode]
+    dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms["sweep_f64_pattern18"]
+    kname = k32name if dom32 else "sweep_f64_pattern18"
+    achieved = kbytes[kname] / (kms[kname] / 1e3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
@@ -244,6 +443,7 @@ def main():
             "omega": args.omega,
             "precision": args.precision,
             "switch_precision": args.switch,
+            "mode": MODE_NAMES[args.mode],
             "check32": args.check32,
             "checkprecision": 1,
             "refresh": args.refresh,
@@ -262,10 +462,11 @@ def main():
         "mixed_switch_1e-6": None if ns is None else {
             "time_to_theta_s": ns.ms_total / 1e3, "sweeps": ns.sweeps, "sweeps_fp32": ns.sweeps_fp32,
             "gff": ns.gff, "speedup_vs_fp64": fp64.ms_total / ns.ms_total},
+        "mixed_modes": modes,
         "site_updates_per_s_fp64": None if fp64 is None else fp64.sweeps * V / (fp64.ms_total / 1e3),
         "sweep_ms_in_run": {"fp32": t32, "fp64": t64},
-        "kernel_ms": {"sweep_f32_r12_gauge": k32, "sweep_f64_pattern18": k64, "dropin_sweep_f64_pattern18": kdrop},
-        "roofline_dropin_f64_frac": bytes64 / (kdrop / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "kernel_ms": kms,
+        "kernel_hbm_frac": {k: kbytes[k] / (kms[k] / 1e3) / 1e9 / HBM_PEAK_GBS for k in kms},
         "roofline": {
             "bound": "hbm",
             "kernel": kname,
@@ -274,10 +475,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": kbytes,
+            "algorithmic_bytes_per_launch": kbytes[kname],
         },
-        "roofline_fp64_kernel_frac": bytes64 / (k64 / 1e3) / 1e9 / HBM_PEAK_GBS,
-        "roofline_fp32_kernel_frac": bytes32 / (k32 / 1e3) / 1e9 / HBM_PEAK_GBS,
         "cpu_baseline": cpu,
         "wall_s": wall,
     }

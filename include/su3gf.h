@@ -49,6 +49,13 @@ typedef enum { GF_LAYOUT_R12 = 12, GF_LAYOUT_PATTERN18 = 18 } gf_layout;
 /* lattice/datatype/lattice_typedefs.h:42 (enum StoppingCrit) */
 typedef enum { GF_CRIT_MAX = 0, GF_CRIT_AVERAGE = 1 } gf_crit;
 
+/* single-precision phase of gf_run_mixed (no reference counterpart) */
+typedef enum {
+    GF_MIXED_LINKS = 0,   /* FP32 links updated together with an FP32 g(x) (any gauge) */
+    GF_MIXED_GAUGE32 = 1, /* g(x) swept on read-only FP32 links, FP32 arithmetic */
+    GF_MIXED_GAUGE64 = 2  /* g(x) swept on read-only FP32 links, FP64 arithmetic */
+} gf_mixed_mode;
+
 typedef struct {
     int dims[4];    /* {Nt, Nx, Ny, Nz} */
     int dtype;      /* gf_dtype */
@@ -162,12 +169,38 @@ typedef struct {
 gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
                     int checkprecision, int reproject, gf_run_result *out, void *stream);
 
+/* Simulated annealing with the reference apps' temperature schedules:
+ *   Landau / Coulomb: `steps` temperatures from tmax down by (tmax-tmin)/steps
+ *     (gaugefixing/apps/LandauGaugeFixingSU3_4D.cu:205-240,
+ *     CoulombGaugeFixingSU3_4D.cu:229-265);
+ *   U1xU1: down to tmin with 15 times finer steps for 2 <= T <= 2.7
+ *     (U1xU1GaugeFixingSU3_4D.cu:212-251);
+ *   MAG: the same with 0.7 <= T <= 0.88 and five heatbath steps per
+ *     temperature (MAGaugeFixingSU3_4D.cu:222-265).
+ * A heatbath step is an SA sweep of both parities (RNG counters *counter,
+ * *counter + 1, ...; *counter is advanced like PhiloxWrapper::getNextCounter)
+ * followed by `micro` microcanonical sweeps; projectSU3 when i % reproject
+ * == 0 for temperature i.  Udw: the slice t-1 buffer of a Coulomb timeslice
+ * (projected together with U), otherwise ignored.  *temperatures (optional)
+ * receives the number of temperatures visited.
+ * mode 0: the reference's sweeps on U in the precision of lat (asynchronous
+ * on `stream`).  mode 1 (FP64 lat; Landau, U1xU1, or Coulomb on a 4D lattice,
+ * every timeslice at once): the gauge transformation is annealed in FP32 on a
+ * read-only FP32 copy of U (same update rules and random numbers, FP32
+ * arithmetic as in the reference's single-precision build), then applied to U
+ * in FP64 and U reprojected; returns when done. */
+gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
+                    int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
+                    void *stream);
+
 /* OR to theta < precision with the quality check fused into the sweep
  * kernels: theta of the odd sites is formed after the odd update, theta of
- * the even sites before the next even update, and the last block of each
- * kernel finishes the reduction in a fixed order, sets a device flag on
- * convergence and makes every later sweep of the run exit at once.  The host
- * reads the flag once per 32 sweeps.  Every sweep is checked (the reference's
+ * the even sites before the next even update; both write per-block partials
+ * that a one-block check kernel reduces in a fixed order.  It sets a device
+ * flag on convergence that makes every later sweep of the run exit at once.
+ * The host reads the flag once per 32 sweeps.  A 4D lattice with
+ * gauge = GF_COULOMB fixes every timeslice at once (spatial links enter the
+ * functional; the time links are transformed from both ends).  Every sweep is checked (the reference's
  * checkprecision = 1); out->sweeps is the index + 1 of the first sweep whose
  * end state met the precision and out->gff / out->theta are that state's; the
  * array then holds that state plus one even half-sweep.  projectSU3 after
@@ -179,24 +212,33 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
 
 /* Average duration of one fused-check sweep kernel (the kernels gf_run_fast
  * and gf_run_mixed launch), from HIP events around nsweeps back-to-back
- * OR sweeps on `stream` (after two warm-up sweeps).  with_gauge selects the
- * FP32 variant that also accumulates a gauge transformation.  U is modified. */
-gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int with_gauge,
+ * OR sweeps on `stream` (after two warm-up sweeps).  variant: 0 the link
+ * sweep in the precision of lat; FP32 only (lat = FP32, 12-real layout):
+ * 1 FP32 links with gauge accumulation, 2 / 3 the gauge-transformation sweep
+ * with FP32 / FP64 arithmetic (U is then the read-only link copy).  U is
+ * modified by variants 0 and 1. */
+gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
 
-/* Mixed precision.  U (FP64, layout of lat) holds the configuration U0 and
- * is left untouched during an FP32 phase that sweeps a compact FP32 copy and
- * accumulates each site's gauge transformation g(x) in FP32; every `refresh`
- * sweeps the FP32 links are rebuilt as G U0 G^dagger from U0 and the FP64
- * projection G of g, so FP32 rounding never alters the gauge orbit.  Once
- * theta < switch_precision (checked every check32 sweeps; the FP32 phase also
- * ends after 50 checks in a row that do not halve its best theta), U <- G(x) U0_mu(x)
- * G(x+mu)^dagger in FP64 and fused-check FP64 sweeps continue to
- * theta < precision, checked every sweep.  No reference
- * counterpart (the reference's DPUPDATES is a compile-time FP32/FP64 mix,
- * gaugefixing/apps/Makefile:46-48). */
+/* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
+ * FP32 phase accumulates each site's gauge transformation g(x) until
+ * theta < switch_precision (checked every check32 sweeps; the phase also ends
+ * after 50 checks in a row that do not halve its best theta); then
+ * U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
+ * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
+ * continue to theta < precision, checked every sweep.
+ *   mode GF_MIXED_LINKS: a compact FP32 copy of the links is swept and g
+ *     updated alongside; every `refresh` sweeps the FP32 links are rebuilt
+ *     as G U0 G^dagger (reprojected every `reproject` sweeps in between).
+ *   mode GF_MIXED_GAUGE32 / GAUGE64: the FP32 copy of U0 is read only and
+ *     the sweep updates g alone (links formed as g U0 g^dagger on the fly),
+ *     in FP32 or FP64 arithmetic; Landau, Coulomb (4D lattice: every
+ *     timeslice at once) and U1xU1.  refresh and reproject are not used in
+ *     this phase.
+ * No reference counterpart (the reference's DPUPDATES is a compile-time
+ * FP32/FP64 mix, gaugefixing/apps/Makefile:46-48). */
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,
-                       float precision, int maxiter, int refresh, int check32, int reproject,
+                       float precision, int maxiter, int mode, int refresh, int check32, int reproject,
                        gf_run_result *out, void *stream);
 
 /* --------------------------------------------------- synthetic inputs

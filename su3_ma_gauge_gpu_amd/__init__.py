@@ -34,6 +34,9 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     LANDAU,
     MAG,
     MICRO,
+    MIXED_GAUGE32,
+    MIXED_GAUGE64,
+    MIXED_LINKS,
     OR,
     PATTERN18,
     R12,
@@ -47,7 +50,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_fast", "run_mixed", "device_info", "RunResult",
+    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "device_info", "RunResult",
 ]
 
 
@@ -397,16 +400,34 @@ def run_fast(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, preci
     return _result(r)
 
 
+def run_sa(lattice: Lattice, U, gauge: int = LANDAU, tmax: float = 0.4, tmin: float = 0.01, steps: int = 1000,
+           micro: int = 3, reproject: int = 100, seed: int = 1, counter: int = 0, Udw=None, mode: int = 0,
+           stream=None) -> tuple[int, int]:
+    """Simulated annealing with the reference apps' schedule for ``gauge``
+    (gf_run_sa in include/su3gf.h); mode 0 sweeps U itself, mode 1 anneals
+    the gauge transformation in FP32 and applies it to U.  Returns (next RNG
+    counter, number of temperatures visited)."""
+    ctr = C.c_uint32(counter)
+    ntemps = C.c_int()
+    _lib.call("gf_run_sa", C.byref(lattice.struct), _ptr(U), _ptr(Udw), gauge, C.c_float(tmax), C.c_float(tmin),
+              int(steps), int(micro), int(reproject), int(mode), int(seed), C.byref(ctr), C.byref(ntemps),
+              _ptr(stream))
+    return int(ctr.value), int(ntemps.value)
+
+
 def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,
-              precision: float = 1e-12, maxiter: int = 20000, refresh: int = 50, check32: int = 10,
-              reproject: int = 100, stream=None) -> RunResult:
-    """FP32 sweeps with an accumulated gauge transformation down to
+              precision: float = 1e-12, maxiter: int = 20000, mode: int | None = None, refresh: int = 50,
+              check32: int = 10, reproject: int = 100, stream=None) -> RunResult:
+    """FP32 sweeps accumulating a gauge transformation down to
     ``switch_precision`` (checked every ``check32`` sweeps), the transformation
     applied to the FP64 configuration, then FP64 fused-check sweeps to
-    ``precision``."""
+    ``precision``.  ``mode``: MIXED_GAUGE64 (default; MIXED_LINKS for MAG),
+    MIXED_GAUGE32 or MIXED_LINKS -- see gf_run_mixed in include/su3gf.h."""
+    if mode is None:
+        mode = MIXED_LINKS if gauge == MAG else MIXED_GAUGE64
     r = _lib.gf_run_result()
     _lib.call("gf_run_mixed", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(switch_precision),
-              C.c_float(precision), int(maxiter), int(refresh), int(check32), int(reproject), C.byref(r),
+              C.c_float(precision), int(maxiter), int(mode), int(refresh), int(check32), int(reproject), C.byref(r),
               _ptr(stream))
     return _result(r)
 

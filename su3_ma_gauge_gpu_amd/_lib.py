@@ -15,6 +15,7 @@ LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
+MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64 = 0, 1, 2
 CRIT_MAX, CRIT_AVERAGE = 0, 1
 
 _STATUS_NAMES = {
@@ -91,7 +92,8 @@ SIGNATURES = {
     "gf_run_or": (_i, [_pl, _vp, _i, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_run_fast": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
-    "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
+    "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_make_config": (_i, [_pi4, _d, _u32, _u32, C.POINTER(_d)]),
 }
 

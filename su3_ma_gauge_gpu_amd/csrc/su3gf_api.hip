@@ -470,6 +470,118 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
 
 }  // extern "C"
 
+// ------------------------------------------------------ simulated annealing
+extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
+                               int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
+                               void *stream)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!U || !counter) return fail(GF_ERR_BAD_ARG, "null argument");
+    if (steps < 0 || micro < 0) return fail(GF_ERR_BAD_ARG, "negative step count");
+    if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
+    if (mode != 0 && mode != 1) return fail(GF_ERR_BAD_ARG, "unknown annealing mode %d", mode);
+    if (mode == 1) {
+        if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "mode 1 takes an FP64 configuration");
+        if (gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
+        if (gauge == GF_COULOMB && lat->dims[0] == 1)
+            return fail(GF_ERR_BAD_ARG, "mode 1 anneals every timeslice of a 4D lattice at once");
+    }
+    hipStream_t s = S(stream);
+    void *dw = gauge == GF_COULOMB ? Udw : U;
+    // mode 1: FP32 copy of U read only, the gauge transformation swept
+    gf_lattice l32 = *lat;
+    l32.dtype = GF_F32;
+    l32.layout = GF_LAYOUT_R12;
+    struct Bufs {
+        void *U32 = nullptr, *g = nullptr;
+        ~Bufs()
+        {
+            if (U32) (void)hipFree(U32);
+            if (g) (void)hipFree(g);
+        }
+    } b;
+    GaugeSweepArgs ga{};
+    const Geom geo = make_geom(lat->dims, V);
+    if (mode == 1) {
+        HIP_TRY(hipMalloc(&b.U32, gf_lattice_bytes(&l32)));
+        HIP_TRY(hipMalloc(&b.g, sizeof(float) * 12 * V));
+        HIP_TRY(launch_gauge_identity((float *)b.g, V, s));
+        HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
+        ga.U0 = (const float *)b.U32;
+        ga.g = (float *)b.g;
+        ga.geo = geo;
+        ga.remap = 1;
+        ga.seed = (uint32_t)seed;
+        ga.nsb = 32;
+    }
+    auto project = [&]() -> gf_status {
+        if (mode == 1) return GF_OK;  // the gauge-transformation sweep keeps g unitary itself
+        GF_TRY(gf_project(lat, U, s));
+        if (gauge == GF_COULOMB && Udw && Udw != U) GF_TRY(gf_project(lat, Udw, s));
+        return GF_OK;
+    };
+    // one heatbath temperature: SA on both parities, then `micro`
+    // microcanonical pairs (LandauGaugeFixingSU3_4D.cu:214-223)
+    auto heat = [&](float T) -> gf_status {
+        if (mode == 1) {
+            for (int par = 0; par < 2; par++) {
+                ga.parity = par;
+                ga.param = T;
+                ga.counter = (*counter)++;
+                HIP_TRY(launch_gsweep(gauge, GF_SA, 0, false, ga, s));
+            }
+            for (int m = 0; m < micro; m++)
+                for (int par = 0; par < 2; par++) {
+                    ga.parity = par;
+                    HIP_TRY(launch_gsweep(gauge, GF_MICRO, 0, false, ga, s));
+                }
+            return GF_OK;
+        }
+        for (int par = 0; par < 2; par++) GF_TRY(gf_sweep(lat, U, dw, par, gauge, GF_SA, T, seed, (*counter)++, 32, s));
+        for (int m = 0; m < micro; m++)
+            for (int par = 0; par < 2; par++) GF_TRY(gf_sweep(lat, U, dw, par, gauge, GF_MICRO, 0.f, 0, 0, 32, s));
+        return GF_OK;
+    };
+    float temperature = tmax;
+    const float base = (tmax - tmin) / (float)steps;
+    int n = 0;
+    if (gauge == GF_LANDAU || gauge == GF_COULOMB) {
+        // fixed number of temperatures (LandauGaugeFixingSU3_4D.cu:205-240,
+        // CoulombGaugeFixingSU3_4D.cu:229-265)
+        for (int i = 0; i < steps; i++, n++) {
+            GF_TRY(heat(temperature));
+            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
+            temperature -= base;
+        }
+    } else {
+        // down to tmin with steps 15 times finer in the critical window:
+        // U1xU1 2 <= T <= 2.7 (U1xU1GaugeFixingSU3_4D.cu:219-251), MAG
+        // 0.7 <= T <= 0.88 with five heatbath steps per temperature
+        // (MAGaugeFixingSU3_4D.cu:228-265); float T against double bounds
+        const double lo = gauge == GF_U1XU1 ? 2.0 : 0.7, hi = gauge == GF_U1XU1 ? 2.7 : 0.88;
+        const int per_t = gauge == GF_MAG ? 5 : 1;
+        int i = 0;
+        do {
+            const float step = (temperature <= hi && temperature >= lo) ? base / 15 : base;
+            for (int j = 0; j < per_t; j++) GF_TRY(heat(temperature));
+            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
+            temperature -= step;
+            i++;
+            n++;
+        } while (temperature >= tmin);
+    }
+    if (mode == 1) {
+        // U <- G U G^dagger in FP64 (G the FP64 projection of g), third rows rebuilt
+        HIP_TRY(launch_project_gauge((float *)b.g, V, s));
+        HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, (const float *)b.g, U, GF_F64, lat->layout, geo, s));
+        GF_TRY(gf_project(lat, U, s));
+        HIP_TRY(hipStreamSynchronize(s));  // before the scratch buffers are freed
+    }
+    if (temperatures) *temperatures = n;
+    return GF_OK;
+}
+
 // ------------------------------------------------- fused-check drivers
 namespace {
 
@@ -506,35 +618,37 @@ constexpr int kChunk = 32;  // sweeps enqueued between two host reads of the dev
 
 double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12.; }
 
-// One OR phase with the check fused into the sweeps.  `after_sweep(it)` runs
-// after sweep `it` (reprojection, refresh).  Sweeps after convergence exit at
-// once.  On return r->theta / r->gff describe the state after the first sweep
-// that met `tol` (the buffer then holds that state plus one even half-sweep),
-// or the final state if maxiter was reached.
+// One OR phase with the check fused into the sweeps.  `launch(parity,
+// theta, partial, it)` enqueues one parity kernel of sweep `it`;
+// `after_sweep(it)` runs after sweep `it` (reprojection, refresh).  Sweeps
+// after convergence exit at once.  On return r->theta / r->gff describe the
+// state after the first sweep that met `tol` (the buffer then holds that
+// state plus one even half-sweep), or the final state if maxiter was reached.
 // The end state of sweep `it` is checked when (it + 1) % check_every == 0:
 // its odd kernel forms the odd-site theta, the next even kernel the even-site
 // theta before updating, and a check kernel completes the sum.  The phase ends
 // once theta < tol or, with stall_limit > 0, after stall_limit checks in a row
 // that do not halve the smallest theta seen (r->converged = 2).
-template <typename R, typename After>
-gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
-                      int check_every, int stall_limit, FusedCtx &fc, hipStream_t s, After after_sweep,
-                      int *sweeps_done, gf_run_result *r)
+// Coulomb on a 4D lattice: each timeslice is its own problem and must meet
+// the precision on its own.  The per-slice check needs the 64-site blocks to
+// tile the slices; otherwise the lattice average is tested.
+int check_segments(const int dims[4], int gauge)
 {
-    int64_t V = 1;
-    for (int i = 0; i < 4; i++) V *= lat->dims[i];
-    const Geom g = make_geom(lat->dims, V);
+    if (gauge != GF_COULOMB || dims[0] == 1) return 1;
+    const int64_t half_slice = (int64_t)dims[1] * dims[2] * dims[3] / 2;
+    return half_slice % 64 == 0 ? dims[0] : 1;
+}
+
+template <typename Launch, typename After>
+gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int maxiter, int check_every,
+                     int stall_limit, int64_t nblocks, FusedCtx &fc, hipStream_t s, Launch launch,
+                     After after_sweep, int *sweeps_done, gf_run_result *r)
+{
+    const int nseg = check_segments(dims, gauge);
     if (check_every < 1) check_every = 1;
+    if (nblocks > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
     hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit, s);
     if (e != hipSuccess) return hip_fail(e, "state init");
-    SweepArgs<R> a{};
-    a.U = U;
-    a.Udw = U;
-    a.gacc = gacc;
-    a.state = fc.st;
-    a.g = g;
-    a.param = omega;
-    a.nsb = 32;
     int it = 0;
     bool done = false;
     while (it < maxiter && !done) {
@@ -542,18 +656,13 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         for (int k = 0; k < n; k++, it++) {
             const bool check_prev = it > 0 && it % check_every == 0;   // end of sweep it-1
             const bool check_this = (it + 1) % check_every == 0;       // end of sweep it
-            a.sweep = it;
-            a.parity = 0;
-            a.partial = fc.even();
-            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, check_prev ? 1 : 0, gacc != nullptr, a, s);
+            e = launch(0, check_prev ? 1 : 0, fc.even(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
             if (check_prev) {
-                e = launch_fused_check(fc.st, fc.even(), fc.odd(), fc.nblocks, it, s);
+                e = launch_fused_check(fc.st, fc.even(), fc.odd(), nblocks, nseg, it, s);
                 if (e != hipSuccess) return hip_fail(e, "check kernel launch");
             }
-            a.parity = 1;
-            a.partial = fc.odd();
-            e = fast::launch_sweep<R>(lat->layout, gauge, GF_OR, check_this ? 2 : 0, gacc != nullptr, a, s);
+            e = launch(1, check_this ? 2 : 0, fc.odd(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
             GF_TRY(after_sweep(it));
         }
@@ -573,10 +682,36 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
     return GF_OK;
 }
 
+// the fused loop over the link sweep kernels (FP64, or FP32 with g accumulation)
+template <typename R, typename After>
+gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
+                      int check_every, int stall_limit, FusedCtx &fc, hipStream_t s, After after_sweep,
+                      int *sweeps_done, gf_run_result *r)
+{
+    int64_t V = 1;
+    for (int i = 0; i < 4; i++) V *= lat->dims[i];
+    SweepArgs<R> a{};
+    a.U = U;
+    a.Udw = U;
+    a.gacc = gacc;
+    a.state = fc.st;
+    a.g = make_geom(lat->dims, V);
+    a.param = omega;
+    a.nsb = 32;
+    auto launch = [&](int parity, int theta, double *partial, int it) {
+        a.sweep = it;
+        a.parity = parity;
+        a.partial = partial;
+        return fast::launch_sweep<R>(lat->layout, gauge, GF_OR, theta, gacc != nullptr, a, s);
+    };
+    return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, stall_limit, sweep_blocks(V / 2), fc, s,
+                      launch, after_sweep, sweeps_done, r);
+}
+
 gf_status final_quality(const gf_lattice *lat, const void *U, int gauge, gf_run_result *r, hipStream_t s)
 {
     gf_stats *st = nullptr;
-    GF_TRY(gf_stats_create(lat->dims, gauge == GF_COULOMB ? GF_LANDAU : gauge, GF_CRIT_AVERAGE, &st));
+    GF_TRY(gf_stats_create(lat->dims, gauge, GF_CRIT_AVERAGE, &st));
     gf_status rc = gf_stats_quality(st, lat, U, &r->gff, &r->theta, s);
     gf_stats_destroy(st);
     return rc;
@@ -592,7 +727,9 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
-    if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_run_fast drives 4D gauges");
+    if (gauge == GF_COULOMB && lat->dims[0] == 1)
+        return fail(GF_ERR_BAD_ARG, "gf_run_fast fixes every timeslice of a 4D lattice at once");
+    if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
     memset(out, 0, sizeof *out);
@@ -625,16 +762,18 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     return GF_OK;
 }
 
-gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int with_gauge, float *ms_per_kernel,
+gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant, float *ms_per_kernel,
                          void *stream)
 {
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
-    if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D gauges");
+    if (variant < 0 || variant > 3) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
+    if (gauge == GF_COULOMB && lat->dims[0] == 1) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D lattices");
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
-    if (with_gauge && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "gauge accumulation is an FP32 variant");
+    if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-3 time FP32 kernels");
+    if (variant >= 2 && gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
     hipStream_t s = S(stream);
     FusedCtx fc;
     GF_TRY(fc.init(V / 2));
@@ -642,7 +781,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         float *p = nullptr;
         ~G() { if (p) (void)hipFree(p); }
     } gb;
-    if (with_gauge) {
+    if (variant > 0) {
         HIP_TRY(hipMalloc((void **)&gb.p, sizeof(float) * 12 * V));
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
@@ -651,25 +790,26 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
+    const Geom geo = make_geom(lat->dims, V);
     auto run = [&](int n, int first) -> gf_status {
         for (int it = 0; it < n; it++) {
-            if (lat->dtype == GF_F64) {
-                SweepArgs<double> a{};
-                a.U = (double *)U; a.Udw = a.U; a.state = fc.st;
-                a.g = make_geom(lat->dims, V); a.param = 1.7f; a.nsb = 32; a.sweep = first + it;
-                for (int par = 0; par < 2; par++) {
-                    a.parity = par;
-                    a.partial = par ? fc.odd() : fc.even();
+            for (int par = 0; par < 2; par++) {
+                double *partial = par ? fc.odd() : fc.even();
+                if (lat->dtype == GF_F64) {
+                    SweepArgs<double> a{};
+                    a.U = (double *)U; a.Udw = a.U; a.state = fc.st; a.g = geo; a.param = 1.7f; a.nsb = 32;
+                    a.sweep = first + it; a.parity = par; a.partial = partial;
                     HIP_TRY(fast::launch_sweep<double>(lat->layout, gauge, GF_OR, par + 1, false, a, s));
-                }
-            } else {
-                SweepArgs<float> a{};
-                a.U = (float *)U; a.Udw = a.U; a.gacc = gb.p; a.state = fc.st;
-                a.g = make_geom(lat->dims, V); a.param = 1.7f; a.nsb = 32; a.sweep = first + it;
-                for (int par = 0; par < 2; par++) {
-                    a.parity = par;
-                    a.partial = par ? fc.odd() : fc.even();
-                    HIP_TRY(fast::launch_sweep<float>(12, gauge, GF_OR, par + 1, with_gauge != 0, a, s));
+                } else if (variant <= 1) {
+                    SweepArgs<float> a{};
+                    a.U = (float *)U; a.Udw = a.U; a.gacc = gb.p; a.state = fc.st; a.g = geo; a.param = 1.7f;
+                    a.nsb = 32; a.sweep = first + it; a.parity = par; a.partial = partial;
+                    HIP_TRY(fast::launch_sweep<float>(12, gauge, GF_OR, par + 1, variant == 1, a, s));
+                } else {
+                    GaugeSweepArgs a{};
+                    a.U0 = (const float *)U; a.g = gb.p; a.state = fc.st; a.partial = partial; a.geo = geo;
+                    a.parity = par; a.param = 1.7f; a.remap = 1;
+                    HIP_TRY(launch_gsweep(gauge, GF_OR, par + 1, variant == 3, a, s));
                 }
             }
         }
@@ -687,14 +827,22 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
 }
 
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision, float precision,
-                       int maxiter, int refresh, int check32, int reproject, gf_run_result *out, void *stream)
+                       int maxiter, int mode, int refresh, int check32, int reproject, gf_run_result *out,
+                       void *stream)
 {
     constexpr int kStall32 = 50;  // FP32 checks without halving theta before switching anyway
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
     if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "the mixed driver takes an FP64 configuration");
-    if (gauge == GF_COULOMB) return fail(GF_ERR_BAD_ARG, "gf_run_mixed drives 4D gauges");
+    if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
+    if (mode < GF_MIXED_LINKS || mode > GF_MIXED_GAUGE64) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
+    if (gauge == GF_COULOMB && lat->dims[0] == 1)
+        return fail(GF_ERR_BAD_ARG, "gf_run_mixed fixes every timeslice of a 4D lattice at once");
+    if (gauge == GF_COULOMB && mode == GF_MIXED_LINKS)
+        return fail(GF_ERR_BAD_ARG, "Coulomb runs with the gauge-transformation modes");
+    if (gauge == GF_MAG && mode != GF_MIXED_LINKS)
+        return fail(GF_ERR_BAD_ARG, "MAG runs with GF_MIXED_LINKS (its update is not linear in the links)");
     memset(out, 0, sizeof *out);
     hipStream_t s = S(stream);
     gf_lattice l32 = *lat;
@@ -726,23 +874,43 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     HIP_TRY(hipEventRecord(ev.a, s));
     HIP_TRY(launch_gauge_identity(g, V, s));
     HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
-    // FP32 phase: links and gauge transformation in FP32; every `refresh`
-    // sweeps the working links are rebuilt from the FP64 configuration and
-    // the (reprojected) transformation, which removes the FP32 drift of the
-    // gauge-invariant content.
-    auto after32 = [&](int it) -> gf_status {
-        const int k = it + 1;
-        if (refresh > 0 && k % refresh == 0) {
-            HIP_TRY(launch_project_gauge(g, V, s));
-            HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, b.U32, GF_F32, 12, geo, s));
-        } else if (reproject > 0 && it % reproject == 0) {
-            HIP_TRY(exact::launch_project<float>(12, 4, (float *)b.U32, V, s));
-        }
-        return GF_OK;
-    };
     int done32 = 0;
-    GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32, kStall32, fc,
-                              s, after32, &done32, out));
+    if (mode == GF_MIXED_LINKS) {
+        // FP32 links and gauge transformation; every `refresh` sweeps the
+        // working links are rebuilt from the FP64 configuration and the
+        // (reprojected) transformation, which removes the FP32 drift of the
+        // gauge-invariant content.
+        auto after32 = [&](int it) -> gf_status {
+            const int k = it + 1;
+            if (refresh > 0 && k % refresh == 0) {
+                HIP_TRY(launch_project_gauge(g, V, s));
+                HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, b.U32, GF_F32, 12, geo, s));
+            } else if (reproject > 0 && it % reproject == 0) {
+                HIP_TRY(exact::launch_project<float>(12, 4, (float *)b.U32, V, s));
+            }
+            return GF_OK;
+        };
+        GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32, kStall32,
+                                  fc, s, after32, &done32, out));
+    } else {
+        // the links stay U0 (FP32 copy, read only); only g is swept, and the
+        // kernel reprojects it in every update
+        GaugeSweepArgs ga{};
+        ga.U0 = (const float *)b.U32;
+        ga.g = g;
+        ga.state = fc.st;
+        ga.geo = geo;
+        ga.param = omega;
+        ga.remap = 1;
+        auto launch = [&](int parity, int theta, double *partial, int) {
+            ga.parity = parity;
+            ga.partial = partial;
+            return launch_gsweep(gauge, GF_OR, theta, mode == GF_MIXED_GAUGE64, ga, s);
+        };
+        auto none = [](int) -> gf_status { return GF_OK; };
+        GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kStall32, gsweep_blocks(V / 2),
+                          fc, s, launch, none, &done32, out));
+    }
     out->sweeps_fp32 = done32;
     // switch: apply the accumulated transformation to the FP64 configuration
     HIP_TRY(launch_project_gauge(g, V, s));

@@ -49,6 +49,26 @@ template <typename R> struct QualityArgs {
 
 inline int64_t sweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 
+// Single-precision sweep on the gauge transformation (su3gf_gauge.hip): U0 is
+// read only, g(x) is updated; the links are g(x) U0_mu(x) g(x+mu)^dagger.
+struct GaugeSweepArgs {
+    const float *U0;   // compact FP32 links (rows 0,1, stride 12)
+    float *g;          // gauge transformation, rows 0,1 (stride 12, one field)
+    DevState *state;   // fused-theta state (or null)
+    double *partial;   // per-block (theta, gff) partials (fused check)
+    Geom geo;
+    int parity;
+    float param;       // OR parameter or SA temperature
+    int remap;         // deal blocks out to the XCDs in contiguous ranges
+    uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
+    int64_t nsb;       // sites per block of the reference launch (RNG key)
+};
+int64_t gsweep_blocks(int64_t Vh);
+// algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
+// 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
+// compute in FP64 on the FP32 data
+hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+
 // exact translation unit (reference order, no FMA contraction)
 namespace exact {
 template <typename R> hipError_t launch_sweep(int stride, int gauge, int algo, const SweepArgs<R> &a, hipStream_t s);
@@ -78,8 +98,10 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
                              hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep
-// `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update)
-hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int sweep,
-                              hipStream_t s);
+// `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update);
+// nseg > 1: the blocks form nseg independent problems (timeslices), theta is
+// the largest of theirs
+hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int nseg,
+                              int sweep, hipStream_t s);
 
 }  // namespace su3gf

@@ -231,57 +231,101 @@ hipError_t launch_state_init(DevState *st, double tol, double theta_scale, doubl
 }
 
 // Fixed-order reduction of the even partials of sweep `sweep` and the odd
-// partials of sweep `sweep - 1`: thread k sums blocks k, k + 1024, ... in
-// order, then a fixed tree.  Sets the convergence flag once theta < tol.
-__global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double *even, const double *odd,
-                                                      int64_t nblocks, int sweep)
+// partials of sweep `sweep - 1`, then the convergence and stagnation tests.
+// With nseg == 1 the whole lattice is one problem: thread k sums blocks k,
+// k + 1024, ... in order, then a fixed tree.  With nseg > 1 the blocks form
+// nseg equal consecutive segments (the timeslices of a Coulomb lattice fixed
+// all at once): wave w reduces segments w, w + 16, ... (lane k sums blocks
+// k, k + 64, ... of the segment, then a fixed shuffle tree), theta is the
+// largest segment theta -- every slice must meet the precision, as when the
+// reference fixes them one after another -- and gff the average.
+constexpr int kMaxSeg = 4096;
+
+__device__ __forceinline__ void check_update(DevState *st, double theta, double gff, int sweep)
 {
-    if (st->done) return;
-    __shared__ double sa[1024], sb[1024], sc[1024];
-    double te = 0, ge = 0, to = 0;
-    for (int64_t k = threadIdx.x; k < nblocks; k += 1024) {
-        te += even[2 * k];
-        ge += even[2 * k + 1];
-        to += odd[2 * k];
+    st->theta = theta;
+    st->gff = gff;
+    st->checked_sweep = sweep - 1;
+    st->checks++;
+    if (theta < 0.5 * st->best) {
+        st->best = theta;
+        st->stall = 0;
+    } else {
+        st->stall++;
     }
-    sa[threadIdx.x] = te;
-    sb[threadIdx.x] = ge;
-    sc[threadIdx.x] = to;
-    __syncthreads();
-    for (int s = 512; s >= 1; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sa[threadIdx.x] += sa[threadIdx.x + s];
-            sb[threadIdx.x] += sb[threadIdx.x + s];
-            sc[threadIdx.x] += sc[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double theta = (sa[0] + sc[0]) * st->theta_scale;
-        st->theta = theta;
-        st->gff = sb[0] * st->gff_scale;
-        st->checked_sweep = sweep - 1;
-        st->checks++;
-        if (theta < 0.5 * st->best) {
-            st->best = theta;
-            st->stall = 0;
-        } else {
-            st->stall++;
-        }
-        if (theta < st->tol) {
-            st->done = 1;
-            st->converged_sweep = sweep - 1;
-        } else if (st->stall_limit > 0 && st->stall >= st->stall_limit) {
-            st->done = 2;
-            st->converged_sweep = sweep - 1;
-        }
+    if (theta < st->tol) {
+        st->done = 1;
+        st->converged_sweep = sweep - 1;
+    } else if (st->stall_limit > 0 && st->stall >= st->stall_limit) {
+        st->done = 2;
+        st->converged_sweep = sweep - 1;
     }
 }
 
-hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int sweep,
-                              hipStream_t s)
+__global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double *even, const double *odd,
+                                                      int64_t nblocks, int nseg, int sweep)
 {
-    hipLaunchKernelGGL(k_fused_check, dim3(1), dim3(1024), 0, s, st, even, odd, nblocks, sweep);
+    if (st->done) return;
+    __shared__ double sa[1024], sb[1024], sc[1024];
+    if (nseg <= 1) {
+        double te = 0, ge = 0, to = 0;
+        for (int64_t k = threadIdx.x; k < nblocks; k += 1024) {
+            te += even[2 * k];
+            ge += even[2 * k + 1];
+            to += odd[2 * k];
+        }
+        sa[threadIdx.x] = te;
+        sb[threadIdx.x] = ge;
+        sc[threadIdx.x] = to;
+        __syncthreads();
+        for (int s = 512; s >= 1; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                sa[threadIdx.x] += sa[threadIdx.x + s];
+                sb[threadIdx.x] += sb[threadIdx.x + s];
+                sc[threadIdx.x] += sc[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) check_update(st, (sa[0] + sc[0]) * st->theta_scale, sb[0] * st->gff_scale, sweep);
+        return;
+    }
+    __shared__ double seg_theta[kMaxSeg];
+    __shared__ double seg_gff[kMaxSeg];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t per = nblocks / nseg;
+    for (int g = wave; g < nseg; g += 16) {
+        double t = 0, f = 0;
+        for (int64_t k = lane; k < per; k += 64) {
+            const int64_t b = (int64_t)g * per + k;
+            t += even[2 * b] + odd[2 * b];
+            f += even[2 * b + 1];
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            t += __shfl_xor(t, off, 64);
+            f += __shfl_xor(f, off, 64);
+        }
+        if (lane == 0) {
+            seg_theta[g] = t;
+            seg_gff[g] = f;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tmax = 0, ftot = 0;
+        for (int g = 0; g < nseg; g++) {
+            tmax = fmax(tmax, seg_theta[g]);
+            ftot += seg_gff[g];
+        }
+        check_update(st, tmax * st->theta_scale * nseg, ftot * st->gff_scale, sweep);
+    }
+}
+
+hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int nseg,
+                              int sweep, hipStream_t s)
+{
+    if (nseg < 1 || nseg > kMaxSeg || nblocks % nseg) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fused_check, dim3(1), dim3(1024), 0, s, st, even, odd, nblocks, nseg, sweep);
     return hipGetLastError();
 }
 

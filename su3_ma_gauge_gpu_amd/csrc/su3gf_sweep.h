@@ -343,10 +343,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     const int64_t site = (active ? p : 0) + (a.parity ? a.g.Vh : 0);
     int c[4], par;
     site_coords(a.g, site, c, par);
+    // Coulomb: the time links are transformed but do not enter the sums.  On a
+    // timeslice ({1,Nx,Ny,Nz}) the down time link is U_0(x, t-1) in the slice
+    // t-1 buffer at the same index (CoulombKernelsSU3.hxx:208-250); on a 4D
+    // lattice (all slices at once) it is the ordinary backward link.
     const bool time_link = (G == GF_COULOMB) && (mu == 0);
-    const int64_t nb = time_link ? site : backward(a.g, c, par, mu);
+    const bool slice_dw = time_link && a.g.N[0] == 1;
+    const int64_t nb = slice_dw ? site : backward(a.g, c, par, mu);
     const LinkRef<R> lu = link_ref(a.U, a.g.V, S, site, mu);
-    const LinkRef<R> ld = link_ref(time_link ? a.Udw : a.U, a.g.V, S, nb, mu);
+    const LinkRef<R> ld = link_ref(slice_dw ? a.Udw : a.U, a.g.V, S, nb, mu);
     M3<R> up, dn;
     if (active) {
         load12(lu, up);

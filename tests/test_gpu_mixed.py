@@ -1,0 +1,143 @@
+"""The single-precision gauge-transformation sweep and the Coulomb lattice
+fixed all timeslices at once, against the FP64 oracle.
+
+The gauge-transformation sweep keeps the links U0 (an FP32 copy) fixed and
+updates g(x); the links it stands for are g(x) U0_mu(x) g(x+mu)^dagger, and
+the drivers apply g to the FP64 configuration when the FP32 phase ends.  A
+few such sweeps therefore track the oracle's FP64 link sweeps to FP32
+accuracy, and a full mixed run lands on the oracle's maximum.
+"""
+import numpy as np
+import pytest
+
+from app_schedule import to_timeslices
+from oracle_lib import COULOMB, LANDAU, OR, U1XU1, rows01
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(gf, dims, sigma=0.3, seed=1, haar=2):
+    return gf.make_config(dims, sigma, seed, haar)
+
+
+@pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6)])
+@pytest.mark.parametrize("gauge", [LANDAU, U1XU1])
+def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
+    """Three gauge-transformation sweeps (run_mixed with maxiter = 3 and a
+    switch threshold never met), applied to U0 and projected, against three
+    FP64 OR sweeps of the oracle followed by projectSU3."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_mixed(gf.Lattice(dims), d, gauge, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=3,
+                       mode=mode, check32=10)
+    assert res.sweeps == res.sweeps_fp32 == 3
+    Uo = U.copy()
+    for _ in range(3):
+        for parity in (0, 1):
+            oracle.sweep(gauge, OR, dims, Uo, parity, 1.7)
+    oracle.project_all(dims, Uo)
+    err = np.abs(rows01(d.download(), V) - rows01(Uo, V)).max()
+    assert err < tol, err
+
+
+def _oracle_coulomb_slices(oracle, dims, U4, precision=1e-12, maxiter=20000):
+    """Each timeslice fixed by the oracle on its own, OR with omega 1.7,
+    checked every sweep (the reference's per-slice loop,
+    CoulombGaugeFixingSU3_4D.cu:198-317, with checkprecision = 1; the time
+    links do not enter the Coulomb functional, so the slices are independent)."""
+    dims3 = (1,) + tuple(dims[1:])
+    S = to_timeslices(U4, dims)
+    out = []
+    for t in range(dims[0]):
+        up = S[t].reshape(-1).copy()
+        dw = S[t - 1].reshape(-1).copy()
+        for it in range(maxiter):
+            for parity in (0, 1):
+                oracle.sweep_coulomb(OR, dims3, up, dw, parity, 1.7)
+            g, th = oracle.quality(COULOMB, dims3, up)
+            if th < precision:
+                break
+        out.append((g, th, it + 1))
+    return out
+
+
+@pytest.mark.parametrize("driver", ["fast", "mixed"])
+def test_coulomb_all_slices_at_once(gf, gpu, oracle, driver):
+    """Coulomb gauge on an 8^3 x 8 lattice, all timeslices in one run (each
+    slice must meet theta < 1e-12), against the oracle fixing slice after
+    slice: every slice's functional to 1e-10, every slice converged, and the
+    plaquette unchanged."""
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    lat = gf.Lattice(dims)
+    if driver == "fast":
+        res = gf.run_fast(lat, d, gf.COULOMB, 1.7, 1e-12, 20000, 100)
+    else:
+        res = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-12, precision=1e-12, maxiter=20000)
+    assert res.converged == 1
+    want = _oracle_coulomb_slices(oracle, dims, U)
+    got = to_timeslices(d.download(), dims)
+    dims3 = (1,) + dims[1:]
+    for t in range(dims[0]):
+        g, th = oracle.quality(COULOMB, dims3, got[t].reshape(-1).copy())
+        assert g == pytest.approx(want[t][0], rel=1e-10), (t, g, want[t])
+        assert th < 1.5e-12, (t, th)
+    assert res.gff == pytest.approx(np.mean([w[0] for w in want]), rel=1e-10)
+
+
+@pytest.mark.parametrize("gauge,kw", [("landau", dict(sasteps=12, samax=0.4, samin=0.01)),
+                                      ("u1xu1", dict(sasteps=10, samax=2.8, samin=1.9)),
+                                      ("mag", dict(sasteps=4, samax=0.4, samin=0.01))])
+def test_run_sa_schedule(gf, gpu, oracle, gauge, kw):
+    """gf_run_sa followed by the reference OR loop (gf_run_or) against the
+    apps' schedule on the oracle: same number of RNG counters drawn, same
+    sweep count, functional to 1e-9 (heatbath steps use log / cos / sin,
+    whose last bit may differ between device and host)."""
+    from app_schedule import GAUGES, Options, run_4d
+
+    dims = (8, 8, 8, 8)
+    U = _cfg(gf, dims)
+    o = Options(randomtrafo=0, srmaxiter=0, ormaxiter=300, precision=1e-12, checkprecision=20, reproject=100, **kw)
+    want = run_4d(oracle, gauge, dims, U, o)[0]
+    g = GAUGES[gauge]
+    d = gf.DeviceBuffer.from_numpy(U)
+    lat = gf.Lattice(dims)
+    ctr, ntemps = gf.run_sa(lat, d, g, o.samax, o.samin, o.sasteps, o.microupdates, o.reproject, o.seed, 0)
+    assert ctr == want["counter"] and ntemps > 0
+    res = gf.run_or(lat, d, g, o.orparameter, o.precision, o.ormaxiter, o.checkprecision, o.reproject)
+    assert res.sweeps == want["or_sweeps"]
+    assert res.gff == pytest.approx(want["gff"], rel=1e-9)
+
+
+def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
+    """Annealing the gauge transformation in FP32 (gf_run_sa mode 1) changes
+    the links only by a gauge transformation applied in FP64: the plaquette
+    is unchanged to 1e-13 and the links stay unitary; the functional rises
+    like the reference-precision annealing's, and the OR loop then converges."""
+    from test_gpu_parity import _plaquette
+    from oracle_lib import to_matrices
+
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims)
+    lat = gf.Lattice(dims)
+    g0, _ = oracle.quality(LANDAU, dims, U.copy())
+    res = {}
+    for mode in (0, 1):
+        d = gf.DeviceBuffer.from_numpy(U)
+        ctr, ntemps = gf.run_sa(lat, d, gf.LANDAU, 1.45, 0.01, 40, 3, 100, 1, 0, mode=mode)
+        assert (ctr, ntemps) == (80, 40)
+        Ua = d.download()
+        M = to_matrices(Ua, V)
+        assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-13
+        res[mode] = oracle.quality(LANDAU, dims, Ua.copy())[0]
+        if mode == 1:
+            assert _plaquette(M, dims) == pytest.approx(_plaquette(to_matrices(U, V), dims), rel=1e-13)
+            r = gf.run_or(lat, d, gf.LANDAU, 1.7, 1e-12, 5000, 20, 100)
+            assert r.converged == 1
+    assert res[0] > g0 + 0.05 and res[1] > g0 + 0.05
+    assert res[1] == pytest.approx(res[0], abs=0.02)

@@ -257,19 +257,21 @@ def test_run_fast_fixed_sweeps(gf, gpu, oracle, gauge):
     assert res.theta == pytest.approx(t_end, rel=1e-6)
 
 
-@pytest.mark.parametrize("dims,sigma", [((8, 8, 8, 8), 0.3), ((12, 12, 12, 12), 0.25)])
-def test_run_mixed_matches_fp64_oracle(gf, gpu, oracle, dims, sigma):
-    """FP32 sweeps with an accumulated gauge transformation, then FP64: the
-    final functional is the FP64 oracle's to 1e-10 (same gauge orbit, same
-    maximum)."""
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("dims,sigma,switch", [((8, 8, 8, 8), 0.3, 1e-6), ((12, 12, 12, 12), 0.25, 1e-6),
+                                               ((8, 8, 8, 8), 0.3, 1e-12)])
+def test_run_mixed_matches_fp64_oracle(gf, gpu, oracle, dims, sigma, switch, mode):
+    """FP32 phase (links + g, or g alone in FP32 / FP64 arithmetic), then
+    FP64: the final functional is the FP64 oracle's to 1e-10 (same gauge
+    orbit, same maximum)."""
     U = _config(gf, dims, sigma, 1, 2)
     d = gf.DeviceBuffer.from_numpy(U)
-    res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=1e-6, precision=1e-12,
-                       maxiter=20000, refresh=50, check32=10, reproject=100)
+    res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=switch, precision=1e-12,
+                       maxiter=20000, mode=mode, refresh=50, check32=10, reproject=100)
     Uo = U.copy()
     n, g, t = oracle.run_or(LANDAU, dims, Uo, 1.7, 1e-12, 20000, 1, 100)
     assert res.converged == 1 and res.theta < 1e-12
-    assert 0 < res.sweeps_fp32 < res.sweeps
+    assert 0 < res.sweeps_fp32 <= res.sweeps
     assert res.gff == pytest.approx(g, rel=1e-10)
     # the result is a gauge transform of the input: the plaquette is unchanged
     V = int(np.prod(dims))
@@ -299,14 +301,15 @@ def _plaquette(M, dims):
     return tot / (6 * V * 3)
 
 
-def test_run_mixed_fp32_stagnation(gf, gpu, oracle):
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_run_mixed_fp32_stagnation(gf, gpu, oracle, mode):
     """With a switch threshold FP32 cannot reach, the FP32 phase ends on
     stagnation and the FP64 phase still converges to the oracle's maximum."""
     dims = (8, 8, 8, 8)
     U = _config(gf, dims, 0.3, 1, 2)
     d = gf.DeviceBuffer.from_numpy(U)
     res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=1e-30, precision=1e-12,
-                       maxiter=20000, refresh=50, check32=5, reproject=100)
+                       maxiter=20000, mode=mode, refresh=50, check32=5, reproject=100)
     Uo = U.copy()
     n, g, t = oracle.run_or(LANDAU, dims, Uo, 1.7, 1e-12, 20000, 1, 100)
     assert res.converged == 1 and res.theta < 1e-12

@@ -36,6 +36,7 @@
 #include "culgt/LandauKernelsSU3.hxx"
 #include "culgt/MAGKernelsSU3.hxx"
 #include "culgt/PhiloxWrapper.hxx"
+#include "culgt/SiteIndex.hxx"
 #include "culgt/U1xU1KernelsSU3.hxx"
 
 namespace {
@@ -107,7 +108,14 @@ template <class GType> int run4d(const Options &o)
     std::vector<Real> U = o.hot ? std::vector<Real>((size_t)V * 72) : make_config(o, dims, V);
     Real *dU;
     HIPCHECK(hipMalloc(&dU, bytes));
-    lat_index_t *dNn = nullptr;  // the drop-in computes neighbours on the fly
+    // the reference's neighbour table (LandauGaugeFixingSU3_4D.cu:107-116);
+    // the drop-in kernels accept it and compute neighbours on the fly
+    SiteIndex<4, FULL_SPLIT> s(HOST_CONSTANTS::SIZE);
+    std::vector<lat_index_t> nn((size_t)s.getLatticeSize() * 8);
+    s.calculateNeighbourTable(nn.data());
+    lat_index_t *dNn;
+    HIPCHECK(hipMalloc(&dNn, nn.size() * sizeof(lat_index_t)));
+    HIPCHECK(hipMemcpy(dNn, nn.data(), nn.size() * sizeof(lat_index_t), hipMemcpyHostToDevice));
     const bool mag = GType::gauge == GF_MAG, u1 = GType::gauge == GF_U1XU1;
     GaugeFixingStats<4, 3, GType, AVERAGE> gaugeStats(dU, HOST_CONSTANTS::SIZE);
     if (o.hot)
@@ -231,6 +239,7 @@ template <class GType> int run4d(const Options &o)
     }
     printf("BEST gff=%.17g\n", bestGff);
     HIPCHECK(hipFree(dU));
+    HIPCHECK(hipFree(dNn));
     return 0;
 }
 

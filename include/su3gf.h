@@ -210,20 +210,22 @@ gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float 
 gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
                       int reproject, gf_run_result *out, void *stream);
 
-/* Average duration of one fused-check sweep kernel (the kernels gf_run_fast
- * and gf_run_mixed launch), from HIP events around nsweeps back-to-back
- * OR sweeps on `stream` (after two warm-up sweeps).  variant: 0 the link
- * sweep in the precision of lat; FP32 only (lat = FP32, 12-real layout):
- * 1 FP32 links with gauge accumulation, 2 / 3 the gauge-transformation sweep
- * with FP32 / FP64 arithmetic (U is then the read-only link copy).  U is
- * modified by variants 0 and 1. */
+/* Average duration of one sweep kernel as the drivers launch it, from HIP
+ * events around nsweeps back-to-back OR sweeps on `stream` (after two
+ * warm-up sweeps).  variant 0: the link sweep in the precision of lat with
+ * the fused check (gf_run_fast and the FP64 phase check every sweep); FP32
+ * only (lat = FP32, 12-real layout), without the check (the FP32 phases check
+ * every check32 sweeps): 1 FP32 links with gauge accumulation, 2 / 3 the
+ * gauge-transformation sweep with FP32 / FP64 arithmetic (U is then the
+ * read-only link copy).  U is modified by variants 0 and 1. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
 
 /* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
  * FP32 phase accumulates each site's gauge transformation g(x) until
  * theta < switch_precision (checked every check32 sweeps; the phase also ends
- * after 50 checks in a row that do not halve its best theta); then
+ * after 8 checks in a row that do not lower its best theta by 10%, i.e. at
+ * the rounding floor of single precision); then
  * U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
  * continue to theta < precision, checked every sweep.

@@ -628,7 +628,7 @@ double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12
 // its odd kernel forms the odd-site theta, the next even kernel the even-site
 // theta before updating, and a check kernel completes the sum.  The phase ends
 // once theta < tol or, with stall_limit > 0, after stall_limit checks in a row
-// that do not halve the smallest theta seen (r->converged = 2).
+// that do not lower the smallest theta seen by 10% (r->converged = 2).
 // Coulomb on a 4D lattice: each timeslice is its own problem and must meet
 // the precision on its own.  The per-slice check needs the 64-site blocks to
 // tile the slices; otherwise the lattice average is tested.
@@ -804,12 +804,13 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
                     SweepArgs<float> a{};
                     a.U = (float *)U; a.Udw = a.U; a.gacc = gb.p; a.state = fc.st; a.g = geo; a.param = 1.7f;
                     a.nsb = 32; a.sweep = first + it; a.parity = par; a.partial = partial;
-                    HIP_TRY(fast::launch_sweep<float>(12, gauge, GF_OR, par + 1, variant == 1, a, s));
+                    HIP_TRY(fast::launch_sweep<float>(12, gauge, GF_OR, variant == 0 ? par + 1 : 0, variant == 1, a,
+                                                      s));
                 } else {
                     GaugeSweepArgs a{};
                     a.U0 = (const float *)U; a.g = gb.p; a.state = fc.st; a.partial = partial; a.geo = geo;
                     a.parity = par; a.param = 1.7f; a.remap = 1;
-                    HIP_TRY(launch_gsweep(gauge, GF_OR, par + 1, variant == 3, a, s));
+                    HIP_TRY(launch_gsweep(gauge, GF_OR, 0, variant == 3, a, s));
                 }
             }
         }
@@ -830,7 +831,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
                        int maxiter, int mode, int refresh, int check32, int reproject, gf_run_result *out,
                        void *stream)
 {
-    constexpr int kStall32 = 50;  // FP32 checks without halving theta before switching anyway
+    constexpr int kStall32 = 8;  // FP32 checks without a 10% gain in theta before switching anyway
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");

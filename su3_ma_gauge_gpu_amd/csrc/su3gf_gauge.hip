@@ -45,7 +45,134 @@ namespace {
 
 constexpr int kGBlock = 256;  // 64 sites x 4 directions
 
-// acc += A B^dagger
+// The complex products below are written as explicit fused multiply-adds:
+// the compiler keeps a * b + c unfused when c is a sum formed elsewhere,
+// which costs a third more instructions in these issue-bound loops.
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// s += a b
+template <typename A> __device__ __forceinline__ void cmac(Cx<A> &s, const Cx<A> &a, const Cx<A> &b)
+{
+    s.x = fma_(a.x, b.x, s.x);
+    s.x = fma_(-a.y, b.y, s.x);
+    s.y = fma_(a.x, b.y, s.y);
+    s.y = fma_(a.y, b.x, s.y);
+}
+
+// s += a conj(b)
+template <typename A> __device__ __forceinline__ void cmacc(Cx<A> &s, const Cx<A> &a, const Cx<A> &b)
+{
+    s.x = fma_(a.x, b.x, s.x);
+    s.x = fma_(a.y, b.y, s.x);
+    s.y = fma_(a.y, b.x, s.y);
+    s.y = fma_(-a.x, b.y, s.y);
+}
+
+template <typename A> __device__ __forceinline__ Cx<A> czero() { return cx((A)0, (A)0); }
+
+// conj(a b - c d)
+template <typename A> __device__ __forceinline__ Cx<A> cross_term(Cx<A> a, Cx<A> b, Cx<A> c, Cx<A> d)
+{
+    A x = a.x * b.x;
+    x = fma_(-a.y, b.y, x);
+    x = fma_(-c.x, d.x, x);
+    x = fma_(c.y, d.y, x);
+    A y = c.x * d.y;
+    y = fma_(c.y, d.x, y);
+    y = fma_(-a.x, b.y, y);
+    y = fma_(-a.y, b.x, y);
+    return cx(x, y);
+}
+
+// row 2 = conj(row0 x row1) without normalising: the rows are unit to the
+// precision they are stored in
+template <typename A> __device__ __forceinline__ void third_row(M3<A> &u)
+{
+    u.e[2][0] = cross_term(u.e[0][1], u.e[1][2], u.e[0][2], u.e[1][1]);
+    u.e[2][1] = cross_term(u.e[0][2], u.e[1][0], u.e[0][0], u.e[1][2]);
+    u.e[2][2] = cross_term(u.e[0][0], u.e[1][1], u.e[0][1], u.e[1][0]);
+}
+
+// Buffer resource over one component-major field of 12 FP32 components per
+// site (rows 0,1): component c of site i is at byte c*4V + 4i.  Loads through
+// it are a per-lane 32-bit byte offset plus a uniform per-component offset,
+// with no 64-bit address arithmetic per load; the hardware checks every
+// access against the field's size.
+struct Field {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t V4;  // bytes per component plane
+};
+__device__ __forceinline__ Field make_field(const float *base, int64_t V)
+{
+    Field f;
+    // word 3: the untyped-access format the gfx9-family buffer instructions expect
+    f.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(48 * V), 0x00020000);
+    f.V4 = (uint32_t)(4 * V);
+    return f;
+}
+
+// rows 0,1 of site byte offset boff
+template <typename A> __device__ __forceinline__ void load_rows01(const Field &f, uint32_t boff, M3<A> &u)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int comp = 2 * (j + 3 * i);
+            const float re =
+                __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)(comp * f.V4), 0));
+            const float im = __builtin_bit_cast(
+                float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((comp + 1) * f.V4), 0));
+            u.e[i][j] = cx((A)re, (A)im);
+        }
+}
+
+template <typename A> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int comp = 2 * (j + 3 * i);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)u.e[i][j].x), f.r, (int)boff,
+                                                  (int)(comp * f.V4), 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)u.e[i][j].y), f.r, (int)boff,
+                                                  (int)((comp + 1) * f.V4), 0);
+        }
+}
+
+// Z = rows 0,1 of X Y^dagger (Y complete), then row 2 of the SU(3) product
+template <typename A> __device__ __forceinline__ void prod_adag(M3<A> &Z, const M3<A> &X, const M3<A> &Y)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            Cx<A> s = czero<A>();
+#pragma unroll
+            for (int j = 0; j < 3; j++) cmacc(s, X.e[i][j], Y.e[k][j]);
+            Z.e[i][k] = s;
+        }
+    third_row(Z);
+}
+
+// Z = rows 0,1 of X Y (Y complete), then row 2 of the SU(3) product
+template <typename A> __device__ __forceinline__ void prod_ab(M3<A> &Z, const M3<A> &X, const M3<A> &Y)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            Cx<A> s = czero<A>();
+#pragma unroll
+            for (int j = 0; j < 3; j++) cmac(s, X.e[i][j], Y.e[j][k]);
+            Z.e[i][k] = s;
+        }
+    third_row(Z);
+}
+
+// acc (+)= X Y^dagger, all nine entries
 template <typename A> __device__ __forceinline__ void mac_adag(M3<A> &acc, const M3<A> &X, const M3<A> &Y)
 {
 #pragma unroll
@@ -54,21 +181,7 @@ template <typename A> __device__ __forceinline__ void mac_adag(M3<A> &acc, const
         for (int k = 0; k < 3; k++) {
             Cx<A> s = acc.e[i][k];
 #pragma unroll
-            for (int j = 0; j < 3; j++) s = s + mul_conj(X.e[i][j], Y.e[k][j]);
-            acc.e[i][k] = s;
-        }
-}
-
-// acc += A B
-template <typename A> __device__ __forceinline__ void mac_ab(M3<A> &acc, const M3<A> &X, const M3<A> &Y)
-{
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            Cx<A> s = acc.e[i][k];
-#pragma unroll
-            for (int j = 0; j < 3; j++) s = s + X.e[i][j] * Y.e[j][k];
+            for (int j = 0; j < 3; j++) cmacc(s, X.e[i][j], Y.e[k][j]);
             acc.e[i][k] = s;
         }
 }
@@ -78,28 +191,88 @@ template <typename A> __device__ __forceinline__ void zero(M3<A> &m)
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) m.e[i][j] = cx((A)0, (A)0);
+        for (int j = 0; j < 3; j++) m.e[i][j] = czero<A>();
 }
 
-// rows 0,1 from the compact FP32 array, row 2 = conj(row0 x row1)
-template <typename A>
-__device__ __forceinline__ void load_su3(const float *base, int64_t V, int64_t idx, int mu, M3<A> &u)
+// rows I,J of u <- G(q) rows I,J, G(q) = [[q0+iq3, q2+iq1], [-q2+iq1, q0-iq3]]
+template <typename A, int I, int J> __device__ __forceinline__ void left_su2(M3<A> &u, const A q[4])
 {
-    const float *q = base + (int64_t)12 * V * mu + idx;
+    const Cx<A> g00 = cx(q[0], q[3]), g01 = cx(q[2], q[1]), g10 = cx(-q[2], q[1]), g11 = cx(q[0], -q[3]);
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int k = 0; k < 3; k++) {
+        Cx<A> a = czero<A>(), b = czero<A>();
+        cmac(a, g00, u.e[I][k]);
+        cmac(a, g01, u.e[J][k]);
+        cmac(b, g10, u.e[I][k]);
+        cmac(b, g11, u.e[J][k]);
+        u.e[I][k] = a;
+        u.e[J][k] = b;
+    }
+}
+
+// entry (r, I) or (r, J) of M G(q)^dagger: new col I = M[:,I] h00 + M[:,J] h10,
+// new col J = M[:,I] h01 + M[:,J] h11 with G(q)^dagger = [[h00, h01], [h10, h11]]
+template <typename A> struct Hd {
+    Cx<A> h00, h01, h10, h11;
+};
+template <typename A> __device__ __forceinline__ Hd<A> su2_dag(const A q[4])
+{
+    Hd<A> h;
+    h.h00 = cx(q[0], -q[3]);
+    h.h01 = cx(-q[2], -q[1]);
+    h.h10 = cx(q[2], -q[1]);
+    h.h11 = cx(q[0], q[3]);
+    return h;
+}
+template <typename A> __device__ __forceinline__ Cx<A> col_i(const Hd<A> &h, Cx<A> mi, Cx<A> mj)
+{
+    Cx<A> s = czero<A>();
+    cmac(s, mi, h.h00);
+    cmac(s, mj, h.h10);
+    return s;
+}
+template <typename A> __device__ __forceinline__ Cx<A> col_j(const Hd<A> &h, Cx<A> mi, Cx<A> mj)
+{
+    Cx<A> s = czero<A>();
+    cmac(s, mi, h.h01);
+    cmac(s, mj, h.h11);
+    return s;
+}
+
+// Gram-Schmidt on rows 0,1 (SU3.hxx:536-583 without the third row)
+template <typename A> __device__ __forceinline__ void project_rows01(M3<A> &u)
+{
+    A n0 = u.e[0][0].x * u.e[0][0].x;
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int comp = 2 * (j + 3 * i);
-            u.e[i][j] = cx((A)q[V * comp], (A)q[V * (comp + 1)]);
-        }
-    u.e[2][0] = conj(u.e[0][1] * u.e[1][2] - u.e[0][2] * u.e[1][1]);
-    u.e[2][1] = conj(u.e[0][2] * u.e[1][0] - u.e[0][0] * u.e[1][2]);
-    u.e[2][2] = conj(u.e[0][0] * u.e[1][1] - u.e[0][1] * u.e[1][0]);
+    for (int k = 0; k < 3; k++) {
+        if (k) n0 = fma_(u.e[0][k].x, u.e[0][k].x, n0);
+        n0 = fma_(u.e[0][k].y, u.e[0][k].y, n0);
+    }
+    n0 = rsq(n0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.e[0][k] = u.e[0][k] * n0;
+    Cx<A> sp = czero<A>();
+#pragma unroll
+    for (int k = 0; k < 3; k++) cmacc(sp, u.e[1][k], u.e[0][k]);
+    A n1 = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        Cx<A> r = u.e[1][k];
+        r.x = fma_(-u.e[0][k].x, sp.x, r.x);
+        r.x = fma_(u.e[0][k].y, sp.y, r.x);
+        r.y = fma_(-u.e[0][k].x, sp.y, r.y);
+        r.y = fma_(-u.e[0][k].y, sp.x, r.y);
+        u.e[1][k] = r;
+        n1 = fma_(r.x, r.x, n1);
+        n1 = fma_(r.y, r.y, n1);
+    }
+    n1 = rsq(n1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.e[1][k] = u.e[1][k] * n1;
 }
 
 // theta_x and Re tr of the up and down links from sum U_up = G P^dagger and
-// sum U_down = Q G^dagger
+// sum U_down = Q G^dagger (G complete)
 template <int G, typename A>
 __device__ __forceinline__ void site_theta(const M3<A> &Gx, const M3<A> &P, const M3<A> &Q, double &th, double &gf)
 {
@@ -126,6 +299,25 @@ __device__ __forceinline__ int64_t logical_block(int remap)
     return (b & 7) * (n >> 3) + (b >> 3);
 }
 
+// The update rule on the subgroup quaternion, with the U1xU1 restriction to
+// (q0, q3) and normalisation (GaugeFixingSubgroupStep.hxx:320-338).
+template <typename A, int G, int ALGO>
+__device__ __forceinline__ void gauge_update(A q[4], float param, Philox &rng)
+{
+    if (G == GF_U1XU1) {
+        q[1] = 0;
+        q[2] = 0;
+    }
+    calc_update<A, ALGO, false>(q, param, rng);
+    if (G == GF_U1XU1) {
+        q[1] = 0;
+        q[2] = 0;
+        const A n = rsq(q[0] * q[0] + q[3] * q[3]);
+        q[0] *= n;
+        q[3] *= n;
+    }
+}
+
 // Work mapping: a block of 256 threads covers 64 sites as four waves, wave mu
 // forming the direction-mu terms g(y) U0_mu(x)^dagger and g(z) U0_mu(z) of its
 // 64 sites (four 12-real loads per lane, all coalesced).  The terms go through
@@ -137,7 +329,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     constexpr int kMats = THETA ? 2 : 1;  // P and Q apart for theta, else their sum
     __shared__ A lds[kMats * 18 * 4 * 64];
     if (a.state && a.state->done) return;  // converged: uniform early exit
-    const int mu = threadIdx.x >> 6;
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
     const int lane = threadIdx.x & 63;
     const int64_t blk = logical_block(a.remap);
     const int64_t p = blk * 64 + lane;
@@ -146,9 +338,11 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     const int64_t V = a.geo.V;
     int c[4], par;
     site_coords(a.geo, site, c, par);
+    const uint32_t site_off = (uint32_t)site * 4u;
+    const Field fg = make_field(a.g, V);
 
     M3<A> Gx;
-    if (mu == 0) load_su3(a.g, V, site, 0, Gx);
+    if (mu == 0) load_rows01(fg, site_off, Gx);
 
     Philox rng;
     if (ALGO == GF_SA || ALGO == GF_RANDOM) {
@@ -162,37 +356,45 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
         // the random transformation does not depend on the links
         if (mu != 0) return;
     } else {
-        M3<A> P, Q;
-        zero(P);
-        if (THETA) zero(Q);
+        // S_mu = (U0_mu(x) g(y)^dagger)^dagger + g(z) U0_mu(z)
+        M3<A> X, Z;
         if (!(G == GF_COULOMB && mu == 0)) {
             M3<A> T, W;
-            const int64_t y = forward(a.geo, c, par, mu);
-            const int64_t z = backward(a.geo, c, par, mu);
-            load_su3(a.g, V, y, 0, T);
-            load_su3(a.U0, V, site, mu, W);
-            mac_adag(P, T, W);
-            load_su3(a.g, V, z, 0, T);
-            load_su3(a.U0, V, z, mu, W);
-            if (THETA) mac_ab(Q, T, W);
-            else mac_ab(P, T, W);
+            const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
+            const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
+            const Field fu = make_field(a.U0 + 12 * V * mu, V);
+            load_rows01(fu, site_off, W);
+            load_rows01(fg, y_off, T);
+            third_row(T);
+            prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
+            load_rows01(fg, z_off, T);
+            load_rows01(fu, z_off, W);
+            third_row(W);
+            prod_ab(Z, T, W);  // g(z) U0_mu(z)
+        } else {
+            zero(X);
+            zero(Z);
         }
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int k = 2 * (3 * i + j);
-                lds[((k + 0) * 4 + mu) * 64 + lane] = P.e[i][j].x;
-                lds[((k + 1) * 4 + mu) * 64 + lane] = P.e[i][j].y;
                 if (THETA) {
-                    lds[((18 + k + 0) * 4 + mu) * 64 + lane] = Q.e[i][j].x;
-                    lds[((18 + k + 1) * 4 + mu) * 64 + lane] = Q.e[i][j].y;
+                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x;
+                    lds[((k + 1) * 4 + mu) * 64 + lane] = -X.e[j][i].y;
+                    lds[((18 + k + 0) * 4 + mu) * 64 + lane] = Z.e[i][j].x;
+                    lds[((18 + k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y;
+                } else {
+                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
+                    lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
                 }
             }
         __syncthreads();
         if (mu != 0) return;
     }
 
+    // P = sum_mu g(y) U0_mu(x)^dagger, Q = sum_mu g(z) U0_mu(z) (theta), or S = P + Q
     M3<A> P, Q;
     if (ALGO != GF_RANDOM) {
 #pragma unroll
@@ -214,55 +416,55 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
                 if (THETA) Q.e[i][j] = cx(w[0], w[1]);
             }
     }
+    third_row(Gx);
 
     double th = 0, gf = 0;
-    if (THETA == 1 && active) site_theta<G>(Gx, P, Q, th, gf);
+    M3<A> G0;  // theta of the state before the update is formed after it (shorter live ranges)
+    if (THETA == 1) G0 = Gx;
 
-    // M = (P + Q) Gx^dagger
+    // M = (P + Q) Gx^dagger; the three subgroup steps read only the entries
+    // of M they need and update only the entries a later step reads
     M3<A> M;
     zero(M);
     if (ALGO != GF_RANDOM) {
         mac_adag(M, P, Gx);
         if (THETA) mac_adag(M, Q, Gx);
     }
-#define SU3GF_GSTEP(I, J)                                                 \
-    {                                                                     \
-        A q[4];                                                           \
-        quat<A, I, J>(M, q);                                              \
-        if (G == GF_U1XU1) {                                              \
-            q[1] = 0;                                                     \
-            q[2] = 0;                                                     \
-        }                                                                 \
-        calc_update<A, ALGO, false>(q, a.param, rng);                     \
-        if (G == GF_U1XU1) {                                              \
-            q[1] = 0;                                                     \
-            q[2] = 0;                                                     \
-            const A n = rsq(q[0] * q[0] + q[3] * q[3]);                   \
-            q[0] *= n;                                                    \
-            q[3] *= n;                                                    \
-        }                                                                 \
-        right_mult_dag<A, I, J>(M, q);                                    \
-        left_mult<A, I, J>(Gx, q);                                        \
+    {   // subgroup (0,2): reads M00 M02 M20 M22; M00 M10 M02 M12 M22 are read later
+        A q[4];
+        quat<A, 0, 2>(M, q);
+        gauge_update<A, G, ALGO>(q, a.param, rng);
+        const Hd<A> h = su2_dag(q);
+        const Cx<A> m00 = col_i(h, M.e[0][0], M.e[0][2]), m10 = col_i(h, M.e[1][0], M.e[1][2]);
+        const Cx<A> m02 = col_j(h, M.e[0][0], M.e[0][2]), m12 = col_j(h, M.e[1][0], M.e[1][2]);
+        const Cx<A> m22 = col_j(h, M.e[2][0], M.e[2][2]);
+        M.e[0][0] = m00; M.e[1][0] = m10; M.e[0][2] = m02; M.e[1][2] = m12; M.e[2][2] = m22;
+        left_su2<A, 0, 2>(Gx, q);
     }
-    SU3GF_GSTEP(0, 2)
-    SU3GF_GSTEP(1, 2)
-    SU3GF_GSTEP(0, 1)
-#undef SU3GF_GSTEP
-
-    project(Gx);  // keep g unitary at the precision it is stored in
-    if (active) {
-        float *q = a.g + site;
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int comp = 2 * (j + 3 * i);
-                q[V * comp] = (float)Gx.e[i][j].x;
-                q[V * (comp + 1)] = (float)Gx.e[i][j].y;
-            }
+    {   // subgroup (1,2): reads M11 M12 M21 M22; M01 M11 are read later
+        A q[4];
+        quat<A, 1, 2>(M, q);
+        gauge_update<A, G, ALGO>(q, a.param, rng);
+        const Hd<A> h = su2_dag(q);
+        const Cx<A> m01 = col_i(h, M.e[0][1], M.e[0][2]), m11 = col_i(h, M.e[1][1], M.e[1][2]);
+        M.e[0][1] = m01; M.e[1][1] = m11;
+        left_su2<A, 1, 2>(Gx, q);
+    }
+    {   // subgroup (0,1): reads M00 M01 M10 M11
+        A q[4];
+        quat<A, 0, 1>(M, q);
+        gauge_update<A, G, ALGO>(q, a.param, rng);
+        left_su2<A, 0, 1>(Gx, q);
     }
 
-    if (THETA == 2 && active) site_theta<G>(Gx, P, Q, th, gf);
+    project_rows01(Gx);  // keep g unitary at the precision it is stored in
+    if (active) store_rows01(fg, site_off, Gx);
+
+    if (THETA == 1 && active) site_theta<G>(G0, P, Q, th, gf);
+    if (THETA == 2 && active) {
+        third_row(Gx);
+        site_theta<G>(Gx, P, Q, th, gf);
+    }
     if (THETA != 0) {
         th = wave64_sum(th);
         gf = wave64_sum(gf);
@@ -308,6 +510,8 @@ int64_t gsweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
 {
+    // 32-bit byte offsets: a buffer of 12 FP32 components per site must stay below 2^31 bytes
+    if (48 * a.geo.V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
     GaugeSweepArgs b = a;
     if (gsweep_blocks(a.geo.Vh) % 8 != 0) b.remap = 0;
     if (b.nsb <= 0) b.nsb = 32;

@@ -247,7 +247,11 @@ __device__ __forceinline__ void check_update(DevState *st, double theta, double 
     st->gff = gff;
     st->checked_sweep = sweep - 1;
     st->checks++;
-    if (theta < 0.5 * st->best) {
+    // stagnation: stall_limit checks in a row without lowering the best theta
+    // by 10%.  A converging sweep gains far more than that per check at the
+    // check cadences used; single-precision sweeps that have reached their
+    // rounding floor gain nothing.
+    if (theta < 0.9 * st->best) {
         st->best = theta;
         st->stall = 0;
     } else {

@@ -1,13 +1,21 @@
-"""The gauge-fixing applications' schedule, run on the CPU oracle.
+"""The gauge-fixing applications' schedule, run on the CPU oracle or on the
+gfx950 library through its Python mirror.
 
 Test infrastructure: this restates, step for step, the per-copy loops of the
 reference applications -- LandauGaugeFixingSU3_4D.cu:184-286,
 U1xU1GaugeFixingSU3_4D.cu:190-300, MAGaugeFixingSU3_4D.cu:199-345 and
-CoulombGaugeFixingSU3_4D.cu:198-310 -- on top of the oracle's sweeps, so that
-the application built on the drop-in headers (su3_ma_gauge_gpu_amd/apps) can
-be compared with it RESULT line by RESULT line.  The RNG launch counter, the
-float temperature arithmetic and the float/double comparisons follow the
-reference exactly, since they decide how many sweeps run.
+CoulombGaugeFixingSU3_4D.cu:198-310 -- so that the application built on the
+drop-in headers (su3_ma_gauge_gpu_amd/apps) can be compared with it RESULT
+line by RESULT line.  The RNG launch counter, the float temperature
+arithmetic and the float/double comparisons follow the reference exactly,
+since they decide how many sweeps run.
+
+Two backends run the same schedule: OracleBackend (the CPU restatement of the
+sweep) and GpuBackend (the library's gf_sweep / gf_project / gf_stats calls,
+the kernels the application runs).  Heatbath schedules are chaotic at the
+level of the last bit of log / cos / sin, so they are compared with the
+application exactly on the GpuBackend and with the oracle only where the
+dynamics contracts.
 """
 from __future__ import annotations
 
@@ -61,18 +69,108 @@ class Counter:
 F = np.float32
 
 
-def run_4d(oracle, gauge: str, dims, U0: np.ndarray, o: Options) -> list[dict]:
+class OracleBackend:
+    """Schedule steps on the CPU oracle; arrays are numpy link arrays."""
+
+    def __init__(self, oracle):
+        self.o = oracle
+
+    def array(self, host):
+        return np.array(host, dtype=self.o.np_real)
+
+    def empty(self, n):
+        return np.zeros(n, dtype=self.o.np_real)
+
+    def host(self, a):
+        return a
+
+    def sweep(self, g, algo, dims, U, parity, param=0.0, seed=0, counter=0):
+        self.o.sweep(g, algo, dims, U, parity, param, seed, counter)
+
+    def sweep_coulomb(self, algo, dims3, up, dw, parity, param=0.0, seed=0, counter=0):
+        self.o.sweep_coulomb(algo, dims3, up, dw, parity, param, seed, counter)
+
+    def project_all(self, dims, U):
+        self.o.project_all(dims, U)
+
+    def set_hot(self, dims, U, seed, counter):
+        self.o.set_hot(dims, U, seed, counter)
+
+    def quality(self, g, dims, U):
+        return self.o.quality(g, dims, U)
+
+
+class GpuBackend:
+    """Schedule steps through the library (the calls the application's
+    headers make); arrays are device buffers of FP64 GpuPattern links."""
+
+    class Arr:
+        def __init__(self, buf, n):
+            self.buf, self.n = buf, n
+
+    def __init__(self, gf):
+        self.gf = gf
+        self._stats = {}
+
+    def _lat(self, dims):
+        return self.gf.Lattice(dims)
+
+    def array(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        return self.Arr(self.gf.DeviceBuffer.from_numpy(host), host.size)
+
+    def empty(self, n):
+        return self.array(np.zeros(n))
+
+    def host(self, a):
+        return a.buf.download(np.float64, a.n)
+
+    def sweep(self, g, algo, dims, U, parity, param=0.0, seed=0, counter=0):
+        import ctypes as C
+        lat = self._lat(dims)
+        self.gf._lib.call("gf_sweep", C.byref(lat.struct), C.c_void_p(U.buf.ptr), C.c_void_p(U.buf.ptr), parity, g,
+                          algo, C.c_float(param), seed, C.c_uint32(counter & 0xFFFFFFFF), 32, None)
+
+    def sweep_coulomb(self, algo, dims3, up, dw, parity, param=0.0, seed=0, counter=0):
+        import ctypes as C
+        lat = self._lat(dims3)
+        self.gf._lib.call("gf_sweep", C.byref(lat.struct), C.c_void_p(up.buf.ptr), C.c_void_p(dw.buf.ptr), parity,
+                          COULOMB, algo, C.c_float(param), seed, C.c_uint32(counter & 0xFFFFFFFF), 32, None)
+
+    def project_all(self, dims, U):
+        self.gf.CommonKernelsSU3(self._lat(dims)).projectSU3(int(np.prod(dims)) // 32, 32, U.buf)
+
+    def set_hot(self, dims, U, seed, counter):
+        self.gf.CommonKernelsSU3(self._lat(dims)).setHot(int(np.prod(dims)) // 32, 32, U.buf, None, seed, counter)
+
+    def quality(self, g, dims, U):
+        key = (g, tuple(dims))
+        if key not in self._stats:
+            self._stats[key] = self.gf.GaugeFixingStats(g, None, self._lat(dims))
+        st = self._stats[key]
+        st.setPointer(U.buf)
+        st.generateGaugeQuality()
+        return st.getCurrentGff(), st.getCurrentA()
+
+
+def as_backend(x):
+    return x if isinstance(x, (OracleBackend, GpuBackend)) else OracleBackend(x)
+
+
+def run_4d(backend, gauge: str, dims, U0: np.ndarray, o: Options) -> list[dict]:
+    oracle = as_backend(backend)
     g = GAUGES[gauge]
     mag, u1 = g == MAG, g == U1XU1
     prec = float(F(o.precision))
     ctr = Counter()
     results = []
-    U = U0.copy()
+    U = oracle.array(U0)
+    host_U, best = U0, 0.0  # the app's host array: the best copy so far is copied back into it
     if o.hotgaugefield:  # once, before the copies; the copies then continue from each other
         oracle.set_hot(dims, U, o.seed, ctr.next())
     for copy in range(o.gaugecopies):
         if not o.hotgaugefield:
-            U = U0.copy()
+            U = oracle.array(host_U)
         if o.randomtrafo:
             for parity in (0, 1):
                 oracle.sweep(g, RANDOM, dims, U, parity, 0.0, o.seed, ctr.next())
@@ -140,7 +238,10 @@ def run_4d(oracle, gauge: str, dims, U0: np.ndarray, o: Options) -> list[dict]:
         if mag:
             gff, theta = oracle.quality(g, dims, U)
         results.append(dict(copy=copy, gff=gff, theta=theta, or_sweeps=or_steps, sr_sweeps=sr_steps,
-                            counter=ctr.n, U=U))
+                            counter=ctr.n, U=oracle.host(U)))
+        if gff > best:  # LandauGaugeFixingSU3_4D.cu:274-283: copy back the better copy
+            best = gff
+            host_U = results[-1]["U"]
     return results
 
 
@@ -160,7 +261,8 @@ def to_timeslices(U4: np.ndarray, dims) -> np.ndarray:
     return out
 
 
-def run_coulomb(oracle, dims, U4: np.ndarray, o: Options) -> list[dict]:
+def run_coulomb(backend, dims, U4: np.ndarray, o: Options) -> list[dict]:
+    oracle = as_backend(backend)
     Nt = dims[0]
     dims3 = (1, dims[1], dims[2], dims[3])
     slices = to_timeslices(U4, dims)
@@ -172,11 +274,11 @@ def run_coulomb(oracle, dims, U4: np.ndarray, o: Options) -> list[dict]:
         best = 0.0
         for copy in range(o.gaugecopies):
             if not o.hotgaugefield:
-                up = slices[t].reshape(-1).copy()
-                dw = slices[tdw].reshape(-1).copy()
+                up = oracle.array(slices[t].reshape(-1))
+                dw = oracle.array(slices[tdw].reshape(-1))
             else:
-                up = np.empty(slices[t].size, dtype=U4.dtype)
-                dw = np.empty(slices[t].size, dtype=U4.dtype)
+                up = oracle.empty(slices[t].size)
+                dw = oracle.empty(slices[t].size)
                 oracle.set_hot(dims3, up, o.seed, ctr.next())
                 oracle.set_hot(dims3, dw, o.seed, ctr.next())
             if o.randomtrafo:
@@ -214,8 +316,8 @@ def run_coulomb(oracle, dims, U4: np.ndarray, o: Options) -> list[dict]:
             results.append(dict(slice=t, copy=copy, gff=gff, theta=theta, or_sweeps=or_steps, counter=ctr.n))
             if gff > best:
                 best = gff
-                slices[t] = up.reshape(72, -1)
-                slices[tdw] = dw.reshape(72, -1)
+                slices[t] = oracle.host(up).reshape(72, -1)
+                slices[tdw] = oracle.host(dw).reshape(72, -1)
     return results
 
 

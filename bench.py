@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--switch", type=float, default=1e-12)
     ap.add_argument("--switch-northstar", type=float, default=1e-6)
     ap.add_argument("--refresh", type=int, default=50)
-    ap.add_argument("--check32", type=int, default=10)
+    ap.add_argument("--check32", type=int, default=5)
     ap.add_argument("--reproject", type=int, default=100)
     ap.add_argument("--maxiter", type=int, default=20000)
     ap.add_argument("--kernel-sweeps", type=int, default=50)

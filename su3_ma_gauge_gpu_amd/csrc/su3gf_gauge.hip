@@ -289,14 +289,22 @@ __device__ __forceinline__ void site_theta(const M3<A> &Gx, const M3<A> &P, cons
     theta_site<G>(s, th, gf);
 }
 
-// Block order: with `remap` the blocks are dealt out so that each of the eight
-// XCDs (which take blocks round-robin) works through one contiguous eighth of
-// the sites, keeping the neighbours' g in that XCD's L2.
-__device__ __forceinline__ int64_t logical_block(int remap)
+// Block order.  The eight XCDs take blocks round-robin (block b runs on XCD
+// b % 8), each with its own L2.  remap = 1 deals the blocks out so that each
+// XCD works through one contiguous eighth of the sites, which keeps the x, y
+// and z neighbours' g in that XCD's L2.  remap = 2 (Nt divisible by 8 and a
+// timeslice of one parity a whole number of 64-site blocks) also walks each
+// XCD's eighth -- Nt/8 timeslices -- with t innermost, so that the t
+// neighbours' g is read from L2 too: in plain order they are a timeslice
+// (V3/2 sites, megabytes of traffic) apart.
+__device__ __forceinline__ int64_t logical_block(int remap, int64_t slice_blocks, int slices_per_xcd)
 {
     const int64_t b = blockIdx.x, n = gridDim.x;
-    if (!remap) return b;
-    return (b & 7) * (n >> 3) + (b >> 3);
+    if (remap == 0) return b;
+    const int64_t xcd = b & 7, k = b >> 3;
+    if (remap == 1) return xcd * (n >> 3) + k;
+    const int64_t t = xcd * slices_per_xcd + k % slices_per_xcd;  // timeslice
+    return t * slice_blocks + k / slices_per_xcd;
 }
 
 // The update rule on the subgroup quaternion, with the U1xU1 restriction to
@@ -331,7 +339,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     if (a.state && a.state->done) return;  // converged: uniform early exit
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
     const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a.remap);
+    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
     const int64_t p = blk * 64 + lane;
     const bool active = p < a.geo.Vh;
     const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
@@ -513,7 +521,16 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
     // 32-bit byte offsets: a buffer of 12 FP32 components per site must stay below 2^31 bytes
     if (48 * a.geo.V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
     GaugeSweepArgs b = a;
-    if (gsweep_blocks(a.geo.Vh) % 8 != 0) b.remap = 0;
+    const int64_t half_slice = a.geo.Vh / a.geo.N[0];  // sites of one parity in a timeslice
+    if (b.remap && a.geo.N[0] % 8 == 0 && half_slice % 64 == 0) {
+        b.remap = 2;
+        b.slice_blocks = half_slice / 64;
+        b.slices_per_xcd = a.geo.N[0] / 8;
+    } else if (b.remap && gsweep_blocks(a.geo.Vh) % 8 == 0) {
+        b.remap = 1;
+    } else {
+        b.remap = 0;
+    }
     if (b.nsb <= 0) b.nsb = 32;
     return fp64_arith ? launch_a<double>(gauge, algo, theta, b, s) : launch_a<float>(gauge, algo, theta, b, s);
 }

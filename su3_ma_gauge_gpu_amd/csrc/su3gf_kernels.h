@@ -59,7 +59,9 @@ struct GaugeSweepArgs {
     Geom geo;
     int parity;
     float param;       // OR parameter or SA temperature
-    int remap;         // deal blocks out to the XCDs in contiguous ranges
+    int remap;         // block order: 0 plain, else chosen by the launcher (XCD-aware)
+    int slices_per_xcd;  // set by the launcher for remap = 2
+    int64_t slice_blocks;
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
 };

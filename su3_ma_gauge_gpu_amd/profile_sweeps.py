@@ -3,8 +3,11 @@ for kernel traces and hardware-counter runs:
 
 * one device-to-device copy of the FP64 18-real configuration (a plain copy
   of known size, 4 * V * 18 * 8 bytes read and written), three times;
-* `--sweeps` fused-check FP32 sweeps with gauge accumulation on the compact
-  FP32 layout (the mixed driver's FP32 kernel);
+* `--sweeps` FP32 gauge-transformation sweeps on read-only compact FP32
+  links, FP32 arithmetic (the mixed driver's default FP32 kernel), and the
+  same with FP64 arithmetic;
+* `--sweeps` FP32 link sweeps with gauge accumulation on the compact FP32
+  layout (the mixed driver's link mode);
 * `--sweeps` fused-check FP64 sweeps on the 18-real layout (the FP64 kernel);
 * `--sweeps` drop-in (reference-order) FP64 sweeps through gf_sweep.
 
@@ -41,8 +44,11 @@ def main() -> None:
     _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(u32.ptr), 0,
               None)
     ms = C.c_float()
-    _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, 1, C.byref(ms), None)
-    print(f"fp32 fused sweep with gauge accumulation: {ms.value:.4f} ms per kernel", flush=True)
+    for variant, what in ((2, "fp32 gauge-transformation sweep"), (3, "fp32 gauge-transformation sweep, fp64 arithmetic"),
+                          (1, "fp32 link sweep with gauge accumulation")):
+        _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, variant,
+                  C.byref(ms), None)
+        print(f"{what}: {ms.value:.4f} ms per kernel", flush=True)
     _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 0, C.byref(ms),
               None)
     print(f"fp64 fused sweep: {ms.value:.4f} ms per kernel", flush=True)

@@ -116,14 +116,15 @@ def dropin_kernel_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int) 
     return ev.stop_ms(stream) / (2 * nsweeps)
 
 
-def cpu_baseline(dims, U_host: np.ndarray, seconds: float) -> dict:
+def cpu_baseline(dims, U_host: np.ndarray, seconds: float, threads: int | None = None) -> dict:
     """The FP64 oracle (the reference sweep restated in C, OpenMP over the sites
-    of a parity) on this host's cores, on a bounded number of sweeps."""
+    of a parity) on this host's cores (or `threads` of them), on a bounded
+    number of sweeps."""
     sys.path.insert(0, str(ROOT / "tests"))
     from oracle_lib import LANDAU, OR, Oracle
 
     oracle = Oracle("f64")
-    cores = len(os.sched_getaffinity(0))
+    cores = threads or len(os.sched_getaffinity(0))
     oracle.set_threads(cores)
     U = U_host.copy()
     V = int(np.prod(dims))
@@ -417,9 +418,10 @@ def main():
         except (ValueError, OSError):
             traffic = None
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
         cpu = cpu_baseline(dims, U0, args.cpu_seconds)
+        cpu1 = cpu_baseline(dims, U0, args.cpu_seconds / 4, threads=1)
 
     line = {
         "metric": "Landau OR site-updates/s + time to theta<1e-12 at 32^4: mixed vs FP64",
@@ -478,6 +480,7 @@ def main():
             "algorithmic_bytes_per_launch": kbytes[kname],
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "wall_s": wall,
     }
     _lib.call("gf_stream_destroy", stream)

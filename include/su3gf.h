@@ -146,7 +146,9 @@ gf_status gf_stats_quality(gf_stats *st, const gf_lattice *lat, const void *U, d
                            void *stream);
 
 /* generateGaugeQualityPerSite (e.g. LandauKernelsSU3.hxx:71-74): per-site
- * gff_x and theta_x into caller-owned device arrays of V doubles each. */
+ * gff_x and theta_x into caller-owned device arrays of V doubles each.
+ * Coulomb on a 4D lattice gives every timeslice's spatial quality (and
+ * gf_stats_quality its average over the slices). */
 gf_status gf_quality_per_site(const gf_lattice *lat, int gauge, const void *U, double *dGff, double *dA,
                               void *stream);
 
@@ -224,8 +226,9 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
 /* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
  * FP32 phase accumulates each site's gauge transformation g(x) until
  * theta < switch_precision (checked every check32 sweeps; the phase also ends
- * after 8 checks in a row that do not lower its best theta by 10%, i.e. at
- * the rounding floor of single precision); then
+ * at the rounding floor of single precision: 8 checks in a row -- 25 for
+ * MAG -- with theta below 1e-9 -- 1e-5 for MAG -- that do not lower its best
+ * theta by 10%); then
  * U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
  * continue to theta < precision, checked every sweep.

@@ -359,8 +359,7 @@ static gf_status quality_launch(const gf_lattice *lat, int gauge, const void *U,
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U) return fail(GF_ERR_BAD_ARG, "U is null");
-    if (gauge == GF_COULOMB && lat->dims[0] != 1)
-        return fail(GF_ERR_BAD_SHAPE, "Coulomb quality acts on a timeslice: dims[0] must be 1");
+    // Coulomb on a 4D lattice: the spatial quality of every timeslice
     const Geom g = make_geom(lat->dims, V);
     hipError_t e;
     if (lat->dtype == GF_F64) {
@@ -618,6 +617,11 @@ constexpr int kChunk = 32;  // sweeps enqueued between two host reads of the dev
 
 double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12.; }
 
+// theta below which a single-precision phase is judged for stagnation: the
+// rounding floor of FP32 sweeps lies near 1e-12 for the squared theta of
+// Landau, Coulomb and U1xU1, and higher for MAG's theta, a sum of |X|
+double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
+
 // One OR phase with the check fused into the sweeps.  `launch(parity,
 // theta, partial, it)` enqueues one parity kernel of sweep `it`;
 // `after_sweep(it)` runs after sweep `it` (reprojection, refresh).  Sweeps
@@ -647,7 +651,8 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
     const int nseg = check_segments(dims, gauge);
     if (check_every < 1) check_every = 1;
     if (nblocks > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
-    hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit, s);
+    hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit,
+                                     stall_floor(gauge), s);
     if (e != hipSuccess) return hip_fail(e, "state init");
     int it = 0;
     bool done = false;
@@ -786,7 +791,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
-    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, s));
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -831,7 +836,9 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
                        int maxiter, int mode, int refresh, int check32, int reproject, gf_run_result *out,
                        void *stream)
 {
-    constexpr int kStall32 = 8;  // FP32 checks without a 10% gain in theta before switching anyway
+    // FP32 checks without a 10% gain in theta (below stall_floor) before
+    // switching anyway; MAG converges more slowly and gets a longer window
+    const int kStall32 = gauge == GF_MAG ? 25 : 8;
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");

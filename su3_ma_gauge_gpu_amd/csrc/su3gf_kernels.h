@@ -17,6 +17,7 @@ struct DevState {
     int checks;           // number of completed checks
     int stall;            // consecutive checks without halving the best theta
     int stall_limit;      // 0: no stagnation exit
+    double stall_below;   // stagnation is only judged for theta below this
     double best;          // smallest theta seen
     double tol;           // theta threshold
     double theta_scale;   // 1 / (Nc * V)
@@ -98,7 +99,7 @@ hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s);
 hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *out, int to, int so, const Geom &geo,
                               hipStream_t s);
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             hipStream_t s);
+                             double stall_below, hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep
 // `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update);
 // nseg > 1: the blocks form nseg independent problems (timeslices), theta is

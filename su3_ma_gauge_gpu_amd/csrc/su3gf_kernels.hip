@@ -206,7 +206,8 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
 }
 
 // ------------------------------------------------------------ driver state
-__global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit)
+__global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
+                             double stall_below)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     st->done = 0;
@@ -215,6 +216,7 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
     st->checks = 0;
     st->stall = 0;
     st->stall_limit = stall_limit;
+    st->stall_below = stall_below;
     st->best = 1e300;
     st->tol = tol;
     st->theta_scale = theta_scale;
@@ -224,9 +226,10 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
 }
 
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             hipStream_t s)
+                             double stall_below, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale, stall_limit);
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale, stall_limit,
+                       stall_below);
     return hipGetLastError();
 }
 
@@ -247,15 +250,18 @@ __device__ __forceinline__ void check_update(DevState *st, double theta, double 
     st->gff = gff;
     st->checked_sweep = sweep - 1;
     st->checks++;
-    // stagnation: stall_limit checks in a row without lowering the best theta
-    // by 10%.  A converging sweep gains far more than that per check at the
-    // check cadences used; single-precision sweeps that have reached their
-    // rounding floor gain nothing.
+    // stagnation: stall_limit checks in a row, all with theta below
+    // stall_below, without lowering the best theta by 10%.  Single-precision
+    // sweeps that have reached their rounding floor gain nothing; above
+    // stall_below theta may still rise for a while (overrelaxation is not
+    // monotone, and a Coulomb lattice tests its worst slice) and is not judged.
     if (theta < 0.9 * st->best) {
         st->best = theta;
         st->stall = 0;
-    } else {
+    } else if (theta < st->stall_below) {
         st->stall++;
+    } else {
+        st->stall = 0;
     }
     if (theta < st->tol) {
         st->done = 1;

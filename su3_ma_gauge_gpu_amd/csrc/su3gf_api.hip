@@ -96,6 +96,44 @@ Geom make_geom(const int dims[4], int64_t V)
 
 inline hipStream_t S(void *stream) { return (hipStream_t)stream; }
 
+// Driver scratch memory, stream-ordered: allocated and freed on the driver's
+// stream (hipMallocAsync / hipFreeAsync), so a driver neither waits for nor
+// stalls work that other streams run meanwhile, and repeated runs reuse the
+// pool instead of the device allocator.
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    Scratch() = default;
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    ~Scratch()
+    {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    hipError_t alloc(size_t bytes, hipStream_t stream)
+    {
+        s = stream;
+        return hipMallocAsync(&p, bytes, stream);
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+// pinned host copy of a driver's device state: one per host thread, kept
+// until the process ends (freeing pinned memory would synchronise the device)
+hipError_t pinned_state(DevState **out)
+{
+    static thread_local DevState *h = nullptr;
+    if (!h) {
+        hipError_t e = hipHostMalloc((void **)&h, sizeof(DevState), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            h = nullptr;
+            return e;
+        }
+    }
+    *out = h;
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct gf_stats {
@@ -492,23 +530,16 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     gf_lattice l32 = *lat;
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
-    struct Bufs {
-        void *U32 = nullptr, *g = nullptr;
-        ~Bufs()
-        {
-            if (U32) (void)hipFree(U32);
-            if (g) (void)hipFree(g);
-        }
-    } b;
+    Scratch bU32, bg;
     GaugeSweepArgs ga{};
     const Geom geo = make_geom(lat->dims, V);
     if (mode == 1) {
-        HIP_TRY(hipMalloc(&b.U32, gf_lattice_bytes(&l32)));
-        HIP_TRY(hipMalloc(&b.g, sizeof(float) * 12 * V));
-        HIP_TRY(launch_gauge_identity((float *)b.g, V, s));
-        HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
-        ga.U0 = (const float *)b.U32;
-        ga.g = (float *)b.g;
+        HIP_TRY(bU32.alloc(gf_lattice_bytes(&l32), s));
+        HIP_TRY(bg.alloc(sizeof(float) * 12 * V, s));
+        HIP_TRY(launch_gauge_identity(bg.as<float>(), V, s));
+        HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, bU32.p, V, 0, s));
+        ga.U0 = bU32.as<const float>();
+        ga.g = bg.as<float>();
         ga.geo = geo;
         ga.remap = 1;
         ga.seed = (uint32_t)seed;
@@ -572,10 +603,11 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     }
     if (mode == 1) {
         // U <- G U G^dagger in FP64 (G the FP64 projection of g), third rows rebuilt
-        HIP_TRY(launch_project_gauge((float *)b.g, V, s));
-        HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, (const float *)b.g, U, GF_F64, lat->layout, geo, s));
+        HIP_TRY(launch_project_gauge(bg.as<float>(), V, s));
+        HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, bg.as<const float>(), U, GF_F64, lat->layout, geo,
+                                   s));
         GF_TRY(gf_project(lat, U, s));
-        HIP_TRY(hipStreamSynchronize(s));  // before the scratch buffers are freed
+        HIP_TRY(hipStreamSynchronize(s));
     }
     if (temperatures) *temperatures = n;
     return GF_OK;
@@ -585,24 +617,20 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
 namespace {
 
 struct FusedCtx {
+    Scratch dev;                // DevState, then 2 * nblocks doubles for each parity
     DevState *st = nullptr;     // device
     DevState *host = nullptr;   // pinned
-    double *partial = nullptr;  // 2 * nblocks doubles for each parity
+    double *partial = nullptr;
     double *even() { return partial; }
     double *odd() { return partial + 2 * nblocks; }
     int64_t nblocks = 0;
-    ~FusedCtx()
-    {
-        if (st) (void)hipFree(st);
-        if (host) (void)hipHostFree(host);
-        if (partial) (void)hipFree(partial);
-    }
-    gf_status init(int64_t Vh)
+    gf_status init(int64_t Vh, hipStream_t s)
     {
         nblocks = sweep_blocks(Vh);
-        HIP_TRY(hipMalloc((void **)&st, sizeof(DevState)));
-        HIP_TRY(hipHostMalloc((void **)&host, sizeof(DevState), hipHostMallocDefault));
-        HIP_TRY(hipMalloc((void **)&partial, sizeof(double) * 4 * nblocks));  // even, odd
+        HIP_TRY(dev.alloc(256 + sizeof(double) * 4 * nblocks, s));  // state, even, odd
+        st = dev.as<DevState>();
+        partial = (double *)((char *)dev.p + 256);
+        HIP_TRY(pinned_state(&host));
         return GF_OK;
     }
     gf_status read(hipStream_t s)
@@ -740,7 +768,7 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     memset(out, 0, sizeof *out);
     hipStream_t s = S(stream);
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2));
+    GF_TRY(fc.init(V / 2, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -781,13 +809,14 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     if (variant >= 2 && gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
     hipStream_t s = S(stream);
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2));
-    struct G {
+    GF_TRY(fc.init(V / 2, s));
+    Scratch gbuf;
+    struct {
         float *p = nullptr;
-        ~G() { if (p) (void)hipFree(p); }
     } gb;
     if (variant > 0) {
-        HIP_TRY(hipMalloc((void **)&gb.p, sizeof(float) * 12 * V));
+        HIP_TRY(gbuf.alloc(sizeof(float) * 12 * V, s));
+        gb.p = gbuf.as<float>();
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
@@ -857,18 +886,14 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
     const Geom geo = make_geom(lat->dims, V);
-    struct Bufs {
-        void *U32 = nullptr, *g = nullptr;
-        ~Bufs()
-        {
-            if (U32) (void)hipFree(U32);
-            if (g) (void)hipFree(g);
-        }
-    } b;
-    HIP_TRY(hipMalloc(&b.U32, gf_lattice_bytes(&l32)));
-    HIP_TRY(hipMalloc(&b.g, sizeof(float) * 12 * V));
+    Scratch sU32, sg;
+    HIP_TRY(sU32.alloc(gf_lattice_bytes(&l32), s));
+    HIP_TRY(sg.alloc(sizeof(float) * 12 * V, s));
+    struct {
+        void *U32, *g;
+    } b{sU32.p, sg.p};
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2));
+    GF_TRY(fc.init(V / 2, s));
     EventPair ev;
     hipEvent_t mid = nullptr;
     HIP_TRY(hipEventCreate(&ev.a));

@@ -156,3 +156,26 @@ def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
             assert r.converged == 1
     assert res[0] > g0 + 0.05 and res[1] > g0 + 0.05
     assert res[1] == pytest.approx(res[0], abs=0.02)
+
+
+def test_run_mixed_mag_links_mode(gf, gpu, oracle):
+    """MAG in the link mode of the mixed driver (FP32 links with g until
+    theta < 1e-6 or stagnation, then FP64 to 1e-10) ends on the FP64
+    oracle's maximum: functional to 1e-10, and the result is a gauge
+    transformation of the input (plaquette to 1e-13)."""
+    from oracle_lib import MAG, to_matrices
+    from test_gpu_parity import _plaquette
+
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_mixed(gf.Lattice(dims), d, gf.MAG, 1.7, switch_precision=1e-6, precision=1e-10, maxiter=20000,
+                       mode=gf.MIXED_LINKS, check32=10)
+    Uo = U.copy()
+    n, g, t = oracle.run_or(MAG, dims, Uo, 1.7, 1e-10, 20000, 1, 100)
+    assert res.converged == 1 and res.theta < 1e-10
+    assert 0 < res.sweeps_fp32 < res.sweeps
+    assert res.gff == pytest.approx(g, rel=1e-10)
+    assert _plaquette(to_matrices(d.download(), V), dims) == pytest.approx(_plaquette(to_matrices(U, V), dims),
+                                                                           rel=1e-13)

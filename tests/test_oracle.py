@@ -298,3 +298,193 @@ def test_run_or_loop_semantics(oracle, gf):
     C = U.copy()
     n0, _, _ = oracle.run_or(LANDAU, dims, C, 1.7, 1e-10, 7, 1, 100)
     assert n0 == 7
+
+
+# ------------------------------------------ pins of the non-Landau branches
+# cyclic permutation: an SU(3) matrix with zero diagonal, so Re tr P = 0
+PERM = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]], dtype=complex)
+
+
+def test_coulomb_quality_is_landau_quality_without_time_links(oracle):
+    """The Coulomb quality of a timeslice (CoulombKernelsSU3.hxx:110-183) is
+    the Landau quality (pinned by the reference-header golden vectors) with
+    the time direction left out.  Embedded in a {2, Nx, Ny, Nz} lattice whose
+    time links are all the same permutation matrix P, the time terms of the
+    Landau per-site values are exactly zero (P - P in Delta, Re tr P = 0 in
+    the functional), so the per-site values agree bit for bit."""
+    from oracle_lib import COULOMB
+    dims3 = (1, 6, 4, 8)
+    V3 = int(np.prod(dims3))
+    up = random_lattice(dims3, 21)
+    dims4 = (2,) + dims3[1:]
+    V4 = int(np.prod(dims4))
+    A = to_matrices(up, V3)
+    M = np.empty((4, V4, 3, 3), dtype=complex)
+    M[0] = PERM
+    idx0 = np.array([oracle.site_index(dims4, (0,) + oracle.site_coords(dims3, i)[1:]) for i in range(V3)])
+    other = to_matrices(random_lattice(dims3, 23), V3)
+    idx1 = np.array([oracle.site_index(dims4, (1,) + oracle.site_coords(dims3, i)[1:]) for i in range(V3)])
+    for mu in (1, 2, 3):
+        M[mu, idx0] = A[mu]
+        M[mu, idx1] = other[mu]
+    g3, a3 = oracle.quality_per_site(COULOMB, dims3, up)
+    g4, a4 = oracle.quality_per_site(LANDAU, dims4, from_matrices(M))
+    assert np.array_equal(g3, g4[idx0]) and np.array_equal(a3, a4[idx0])
+
+
+def _slice_plaquettes(oracle, dims3, up, dw, nxt):
+    """Traces of the spatial plaquettes of slice t (up) and of the temporal
+    plaquettes between slices t-1 (dw) and t, and between t and t+1 (nxt)."""
+    V3 = int(np.prod(dims3))
+    Mu, Md, Mn = to_matrices(up, V3), to_matrices(dw, V3), to_matrices(nxt, V3)
+    fw = {}
+    for i in range(V3):
+        c = oracle.site_coords(dims3, i)
+        for mu in (1, 2, 3):
+            d = list(c)
+            d[mu] = (d[mu] + 1) % dims3[mu]
+            fw[(i, mu)] = oracle.site_index(dims3, tuple(d))
+    dag = lambda m: np.conj(m.T)
+    out = []
+    for i in range(V3):
+        for mu in (1, 2, 3):
+            for nu in range(mu + 1, 4):
+                P = Mu[mu, i] @ Mu[nu, fw[(i, mu)]] @ dag(Mu[mu, fw[(i, nu)]]) @ dag(Mu[nu, i])
+                out.append(np.trace(P))
+            # (t-1, t): U_i(x, t-1) U_0(x+i, t-1) U_i(x, t)^dag U_0(x, t-1)^dag
+            P = Md[mu, i] @ Md[0, fw[(i, mu)]] @ dag(Mu[mu, i]) @ dag(Md[0, i])
+            out.append(np.trace(P))
+            # (t, t+1): U_i(x, t) U_0(x+i, t) U_i(x, t+1)^dag U_0(x, t)^dag
+            P = Mu[mu, i] @ Mu[0, fw[(i, mu)]] @ dag(Mn[mu, i]) @ dag(Mu[0, i])
+            out.append(np.trace(P))
+    return np.array(out)
+
+
+def test_coulomb_sweeps(oracle):
+    """Coulomb sweeps of a timeslice:
+    * they are a gauge transformation of slice t that also multiplies the time
+      links of slice t from the left and those of slice t-1 from the right
+      (CoulombKernelsSU3.hxx:208-250): every spatial plaquette of slice t and
+      every temporal plaquette to slices t-1 and t+1 keeps its trace;
+    * microcanonical steps conserve the Coulomb (spatial) functional exactly,
+      and omega = 1 OR never lowers it -- both fail if the time links entered
+      the subgroup sum."""
+    from oracle_lib import COULOMB
+    dims3 = (1, 6, 4, 8)
+    up = random_lattice(dims3, 31)
+    dw = random_lattice(dims3, 32)
+    nxt = random_lattice(dims3, 33)
+    p0 = _slice_plaquettes(oracle, dims3, up, dw, nxt)
+    for _ in range(3):
+        for parity in (0, 1):
+            oracle.sweep_coulomb(OR, dims3, up, dw, parity, 1.7)
+    oracle.project_all(dims3, up)
+    oracle.project_all(dims3, dw)
+    assert np.abs(_slice_plaquettes(oracle, dims3, up, dw, nxt) - p0).max() < 1e-13
+    g0, t0 = oracle.quality(COULOMB, dims3, up)
+    for _ in range(4):
+        for parity in (0, 1):
+            oracle.sweep_coulomb(MICRO, dims3, up, dw, parity, 0.0)
+    g1, t1 = oracle.quality(COULOMB, dims3, up)
+    assert g1 == pytest.approx(g0, abs=1e-13) and t1 != pytest.approx(t0, rel=1e-3)
+    g_prev = g1
+    for _ in range(4):
+        for parity in (0, 1):
+            oracle.sweep_coulomb(OR, dims3, up, dw, parity, 1.0)
+        g, _ = oracle.quality(COULOMB, dims3, up)
+        assert g >= g_prev - 1e-14
+        g_prev = g
+    assert g_prev > g1 + 1e-3
+
+
+@pytest.mark.parametrize("algo,param", [(OR, 1.7), (MICRO, 0.0), (OR, 1.0)])
+def test_u1xu1_sweeps_are_diagonal_rotations(oracle, algo, param):
+    """U1xU1 restricts every subgroup element to (q0, 0, 0, q3), a diagonal
+    phase rotation (GaugeFixingSubgroupStep.hxx:238-353): the moduli |U_ij|
+    of every link are unchanged."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = random_lattice(dims, 24)
+    oracle.project_all(dims, U)
+    m0 = np.abs(to_matrices(U, V))
+    for _ in range(3):
+        for parity in (0, 1):
+            oracle.sweep(U1XU1, algo, dims, U, parity, param)
+    oracle.project_all(dims, U)
+    assert np.abs(np.abs(to_matrices(U, V)) - m0).max() < 1e-14
+    # and the links did change
+    assert np.abs(to_matrices(U, V) - to_matrices(random_lattice(dims, 24), V)).max() > 1e-3
+
+
+@pytest.mark.parametrize("gauge", [U1XU1, MAG])
+def test_omega1_or_maximises_each_functional(oracle, gauge):
+    """With omega = 1 the U1xU1 step maximises the Landau functional over
+    diagonal rotations and the MAG step, g0 = D + sqrt(D^2 + E^2 + F^2), the
+    MAG functional sum |U_ii|^2 over the subgroup
+    (GaugeFixingSubgroupStep.hxx:529-575): neither functional decreases."""
+    dims = (4, 6, 4, 8)
+    U = random_lattice(dims, 25)
+    qual = MAG if gauge == MAG else LANDAU
+    g_prev, _ = oracle.quality(qual, dims, U)
+    g_first = g_prev
+    for _ in range(6):
+        for parity in (0, 1):
+            oracle.sweep(gauge, OR, dims, U, parity, 1.0)
+        g, _ = oracle.quality(qual, dims, U)
+        assert g >= g_prev - 1e-14
+        g_prev = g
+    assert g_prev > g_first + 1e-3
+
+
+def _local_max_check(oracle, gauge, functional, generators):
+    """After an omega = 1 OR half-sweep every updated site's last subgroup
+    step, (0,1), left the local functional at a maximum over that subgroup:
+    small extra rotations exp(i eps sigma_k) in (0,1) -- applied to the up
+    links from the left and to the down links from the right -- do not raise
+    it.  A wrong term, sign or factor in the subgroup sum moves the update off
+    the maximum and some rotation raises the functional at first order."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = random_lattice(dims, 41)
+    oracle.sweep(gauge, OR, dims, U, 0, 1.0)
+    oracle.project_all(dims, U)
+    M = to_matrices(U, V)
+    nn = oracle.neighbour_table(dims).reshape(8, V)
+    even = np.arange(V // 2)
+    up = M[:, even]                                        # [mu, site, 3, 3]
+    dn = np.stack([M[mu, nn[2 * mu, even]] for mu in range(4)])
+    base = functional(up, dn)
+    sig = {1: np.array([[0, 1], [1, 0]]), 2: np.array([[0, -1j], [1j, 0]]), 3: np.array([[1, 0], [0, -1]])}
+    worst = -np.inf
+    for k in generators:
+        for eps in (1e-3, -1e-3):
+            R = np.eye(3, dtype=complex)
+            R[:2, :2] = np.cos(eps) * np.eye(2) + 1j * np.sin(eps) * sig[k]
+            f = functional(R @ up, dn @ np.conj(R.T))
+            worst = max(worst, (f - base).max())
+    assert worst < 1e-12, worst
+
+
+def test_mag_update_is_local_maximum(oracle):
+    """MAG (GaugeFixingSubgroupStep.hxx:354-575): the functional is the sum of
+    |U_ii|^2 over the site's eight links; the subgroup step excludes sigma_3
+    (q3 = 0), which leaves it unchanged."""
+    def f(up, dn):
+        d = np.abs(np.diagonal(up, axis1=-2, axis2=-1)) ** 2 + np.abs(np.diagonal(dn, axis1=-2, axis2=-1)) ** 2
+        return d.sum(axis=(0, 2))
+    _local_max_check(oracle, MAG, f, (1, 2))
+
+
+def test_u1xu1_update_is_local_maximum(oracle):
+    """U1xU1: the Landau functional Re tr over the eight links, maximised over
+    the diagonal rotations (sigma_3) alone."""
+    def f(up, dn):
+        return (np.trace(up, axis1=-2, axis2=-1).real + np.trace(dn, axis1=-2, axis2=-1).real).sum(axis=0)
+    _local_max_check(oracle, U1XU1, f, (3,))
+
+
+def test_landau_update_is_local_maximum(oracle):
+    """Landau: the same functional over all three generators."""
+    def f(up, dn):
+        return (np.trace(up, axis1=-2, axis2=-1).real + np.trace(dn, axis1=-2, axis2=-1).real).sum(axis=0)
+    _local_max_check(oracle, LANDAU, f, (1, 2, 3))

@@ -29,6 +29,13 @@
  * the reference source text.  CUDA's rsqrt is restated as 1/sqrt; cospi and
  * sinpi are restated with an exact argument reduction (glibc has neither).
  * Build twice: the default is Real = double; -DORC_F32 gives Real = float.
+ *
+ * Pins (tests/test_oracle.py): golden vectors written from the reference's
+ * own headers, Random123 known answers, closed forms and invariants of every
+ * gauge's sweep and quality, and sampling-distribution tests of the random
+ * update rules.  Parity unpinned: the last ulp of log, cospi and sinpi in the
+ * heatbath and the random transformation (the reference takes them from
+ * CUDA's math library, which is not part of its sources).
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -499,6 +506,17 @@ static void calculate_update(const orc_algo *al, Real a[4])
     case A_RANDOM: upd_random(a, al->rng); break;
     default: break;
     }
+}
+
+/* One update rule on a subgroup quaternion, with the generator of thread
+ * `tid` (key (tid, seed), counter `counter`): what calculateUpdate returns
+ * for that thread (gaugefixing/algorithms/{Or,Micro,Sa,Sr,Random}Update.hxx). */
+ORC_EXPORT void orc_update(int algo, Real a[4], float param, int tid, int seed, uint32_t counter)
+{
+    orc_rng rng;
+    rng_init(&rng, tid, seed, counter);
+    orc_algo al = {algo, param, &rng};
+    calculate_update(&al, a);
 }
 
 /* ------------------------------------------------------------------ */

@@ -67,6 +67,7 @@ class Oracle:
         L.orc_quality.argtypes = [C.c_int, C.c_int * 4, P, pd, pd]
         L.orc_run_or.argtypes = [C.c_int, C.c_int * 4, P, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, pd, pd]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_update.argtypes = [C.c_int, P, C.c_float, C.c_int, C.c_int, C.c_uint32]
 
     def _p(self, arr: np.ndarray):
         assert arr.dtype == self.np_real and arr.flags.c_contiguous
@@ -153,6 +154,12 @@ class Oracle:
         n = self.lib.orc_run_or(gauge, dims_arr(dims), self._p(U), C.c_float(omega), C.c_float(precision), maxiter,
                                 checkprecision, reproject, C.byref(g), C.byref(t))
         return n, g.value, t.value
+
+    def update(self, algo, a, param=0.0, tid=0, seed=0, counter=0):
+        """One update rule applied to the quaternion a (returns the new one)."""
+        a = np.array(a, dtype=self.np_real)
+        self.lib.orc_update(algo, self._p(a), C.c_float(param), tid, seed, C.c_uint32(counter & 0xFFFFFFFF))
+        return a
 
     def set_threads(self, n: int) -> None:
         self.lib.orc_set_threads(n)

@@ -12,11 +12,14 @@
    the functional under omega = 1 OR, exact conservation under microcanonical
    steps, gauge invariance of the plaquette, and the known answer of a pure
    gauge (gff -> 1, theta -> 0).
+5. The random update rules (heatbath, random transformation, stochastic
+   relaxation) one quaternion at a time: Kolmogorov-Smirnov tests against their
+   closed-form sampling densities, with power checks, and exact outcomes.
 """
 import numpy as np
 import pytest
 
-from oracle_lib import LANDAU, MAG, MICRO, OR, U1XU1, from_matrices, rows01, to_matrices
+from oracle_lib import LANDAU, MAG, MICRO, OR, RANDOM, SA, SR, U1XU1, from_matrices, rows01, to_matrices
 
 GOLDEN = None
 
@@ -488,3 +491,143 @@ def test_landau_update_is_local_maximum(oracle):
     def f(up, dn):
         return (np.trace(up, axis1=-2, axis2=-1).real + np.trace(dn, axis1=-2, axis2=-1).real).sum(axis=0)
     _local_max_check(oracle, LANDAU, f, (1, 2, 3))
+
+
+# ---------------------------------------------------------------- update rules
+# Each rule acts on the subgroup quaternion a accumulated for a site; the local
+# functional of a unit quaternion g is the 4D dot product g.a, maximised by
+# g = a/|a| (the omega = 1 OR result).  The random rules are pinned through
+# their sampling distributions (Kolmogorov-Smirnov against the closed-form
+# densities) with a power check that a wrong parameter is rejected.
+
+def _hamilton(p, q):
+    p0, p1, p2, p3 = p
+    q0, q1, q2, q3 = q
+    return np.array([p0 * q0 - p1 * q1 - p2 * q2 - p3 * q3,
+                     p0 * q1 + p1 * q0 + p2 * q3 - p3 * q2,
+                     p0 * q2 - p1 * q3 + p2 * q0 + p3 * q1,
+                     p0 * q3 + p1 * q2 - p2 * q1 + p3 * q0])
+
+
+def _rand_quats(n, seed, scale=2.0):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=(n, 4)) * scale
+
+
+def _samples(oracle, algo, a, param, n, seed=11, counter=5):
+    return np.array([oracle.update(algo, a, param, tid, seed, counter) for tid in range(n)])
+
+
+def _ks_pvalue(x, cdf):
+    from scipy import stats
+    return stats.kstest(x, cdf).pvalue
+
+
+def _kp_cdf(k):
+    """CDF of x on [-1, 1] with density proportional to sqrt(1 - x^2) exp(k x):
+    the SU(2) heatbath distribution of g.a/|a| at beta |a| = k."""
+    from scipy.integrate import cumulative_trapezoid
+    xs = np.linspace(-1.0, 1.0, 400001)
+    w = np.sqrt(np.clip(1.0 - xs * xs, 0.0, None)) * np.exp(k * (xs - 1.0))
+    c = cumulative_trapezoid(w, xs, initial=0.0)
+    c /= c[-1]
+    return lambda x: np.interp(x, xs, c)
+
+
+def test_micro_update_is_reflection_through_optimum(oracle):
+    """MicroUpdate.hxx:43-66: g' = (a/|a|)^2, the reflection of g = 1 through the
+    maximiser; it keeps the local functional (g'.a = a0) and is a unit quaternion."""
+    for a in _rand_quats(200, 3):
+        g = oracle.update(MICRO, a)
+        ah = a / np.linalg.norm(a)
+        np.testing.assert_allclose(g, _hamilton(ah, ah), atol=1e-14)
+        assert abs(g @ a - a[0]) < 1e-13 * np.linalg.norm(a)
+
+
+def test_or_update_closed_forms(oracle):
+    """OrUpdate.hxx:52-78: unit result whose vector part is parallel to a's;
+    omega = 1 gives a/|a|; to first order in the angle the rotation angle of
+    a/|a| is multiplied by omega (the overrelaxation g -> g^omega)."""
+    for a in _rand_quats(100, 4):
+        np.testing.assert_allclose(oracle.update(OR, a, 1.0), a / np.linalg.norm(a), atol=1e-15)
+        for w in (1.3, 1.7, 1.95):
+            g = oracle.update(OR, a, w)
+            assert abs(np.linalg.norm(g) - 1.0) < 1e-14
+            v, u = g[1:], a[1:]
+            np.testing.assert_allclose(np.cross(v, u), 0.0, atol=1e-13)
+            assert v @ u > 0 and np.sign(g[0]) == np.sign(a[0])
+    for w in (1.3, 1.7, 1.95):
+        for th in (1e-3, 3e-3):
+            n = np.array([0.3, -0.5, 0.8]); n /= np.linalg.norm(n)
+            a = 1.7 * np.concatenate([[np.cos(th)], np.sin(th) * n])
+            g = oracle.update(OR, a, w)
+            ang = np.arctan2(np.linalg.norm(g[1:]), g[0])
+            assert abs(ang / th - w) < 2 * th, (w, th, ang / th)
+
+
+@pytest.mark.parametrize("norm,temperature", [(2.0, 4.0), (2.0, 1.0), (3.0, 0.375)])
+def test_sa_update_is_heatbath(oracle, norm, temperature):
+    """SaUpdate.hxx:52-147 (Kennedy-Pendleton): g.a/|a| follows
+    sqrt(1 - x^2) exp(|a| x / T), the component of g orthogonal to a/|a| points
+    uniformly over the 2-sphere, and g is a unit quaternion.  The key (tid) runs
+    over 0..n-1, one independent stream per sample."""
+    n = 12000
+    a = np.array([0.4, -0.7, 0.5, 0.3])
+    a = a / np.linalg.norm(a) * norm
+    ah = a / norm
+    g = _samples(oracle, SA, a, temperature, n)
+    np.testing.assert_allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-13)
+    x = g @ ah
+    k = norm / temperature
+    assert _ks_pvalue(x, _kp_cdf(k)) > 1e-3
+    # power: a 50% error in beta |a| is rejected
+    assert _ks_pvalue(x, _kp_cdf(1.5 * k)) < 1e-6
+    assert _ks_pvalue(x, _kp_cdf(k / 1.5)) < 1e-6
+    # isotropy of the orthogonal part
+    basis = np.linalg.qr(np.column_stack([ah, np.eye(4)[:, :3]]))[0][:, 1:]
+    v = g @ basis
+    u = v / np.linalg.norm(v, axis=1, keepdims=True)
+    assert np.abs(u.mean(axis=0)).max() < 5 / np.sqrt(3 * n)
+    np.testing.assert_allclose(u.T @ u / n, np.eye(3) / 3, atol=5 * np.sqrt(2 / 15 / n))
+
+
+def test_random_update_distribution(oracle):
+    """RandomUpdate.hxx:47-59: the result does not depend on a; a0 = cos(pi alpha)
+    with alpha uniform on [0, 1), and the vector part's direction is uniform on
+    the 2-sphere (cos theta uniform on [-1, 1], phi uniform on [0, 2 pi))."""
+    from scipy import stats
+    n = 12000
+    g = _samples(oracle, RANDOM, np.array([1.0, 0.2, -0.3, 0.4]), 0.0, n)
+    g2 = _samples(oracle, RANDOM, np.array([-0.1, 2.0, 0.5, -0.6]), 0.0, 50)
+    np.testing.assert_array_equal(g[:50], g2)
+    np.testing.assert_allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-14)
+    alpha = np.arccos(np.clip(g[:, 0], -1, 1)) / np.pi
+    assert stats.kstest(alpha, "uniform").pvalue > 1e-3
+    assert stats.kstest(alpha ** 1.1, "uniform").pvalue < 1e-6
+    s = np.linalg.norm(g[:, 1:], axis=1)
+    ct = g[:, 3] / s
+    assert stats.kstest(ct, stats.uniform(-1, 2).cdf).pvalue > 1e-3
+    phi = np.mod(np.arctan2(g[:, 2], g[:, 1]), 2 * np.pi) / (2 * np.pi)
+    assert stats.kstest(phi, "uniform").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("p", [0.0, 0.3, 1.0])
+def test_sr_update_flip_frequency(oracle, p):
+    """SrUpdate.hxx:54-96 (stochastic relaxation): with probability p the result
+    is the normalised square a^2/|a|^2 (the overrelaxed step), otherwise
+    a/|a|; the first draw of the thread's stream decides."""
+    n = 6000
+    quats = _rand_quats(n, 9)
+    flips = 0
+    for tid, a in enumerate(quats):
+        g = oracle.update(SR, a, p, tid, 3, 17)
+        ah = a / np.linalg.norm(a)
+        sq = _hamilton(ah, ah)
+        if np.abs(g - sq).max() < 1e-13:
+            flips += 1
+        else:
+            np.testing.assert_allclose(g, ah, atol=1e-14)
+    if p in (0.0, 1.0):
+        assert flips == int(p) * n
+    else:
+        assert abs(flips / n - p) < 5 * np.sqrt(p * (1 - p) / n), flips / n

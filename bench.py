@@ -39,7 +39,7 @@ import su3_ma_gauge_gpu_amd as gf  # noqa: E402
 from su3_ma_gauge_gpu_amd import _lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-MODE_NAMES = {0: "links_f32+g", 1: "gauge_f32", 2: "gauge_f32_f64arith"}
+MODE_NAMES = {0: "links_f32+g", 1: "gauge_f32", 2: "gauge_f32_f64arith", 3: "gauge_q24"}
 
 
 def parse():
@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--mode", type=int, default=1, help="0 links+g, 1 g (FP32 arithmetic), 2 g (FP64 arithmetic)")
+    ap.add_argument("--mode", type=int, default=3, help="0 links+g, 1 g (FP32 arithmetic), 2 g (FP64 arithmetic), 3 g on 24-bit fixed-point links")
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--sa-steps", type=int, default=3000)
     ap.add_argument("--sa-mode", type=int, default=1, help="0 FP64 link sweeps, 1 FP32 gauge-transformation sweeps")
@@ -385,7 +385,7 @@ def main():
     if not args.no_fp64:
         ns = mixed(args.mode, args.switch_northstar)
         print(f"bench: mixed, switch {args.switch_northstar:g}: {ns}", file=sys.stderr, flush=True)
-        for m in (0, 1, 2):
+        for m in (0, 1, 2, 3):
             rm = r if m == args.mode else mixed(m, args.switch)
             modes[MODE_NAMES[m]] = {"time_to_theta_s": rm.ms_total / 1e3, "sweeps": rm.sweeps,
                                     "sweeps_fp32": rm.sweeps_fp32, "gff": rm.gff,
@@ -400,7 +400,8 @@ def main():
                   C.c_void_p(U32.ptr), 0, stream)
 
     kms = {}
-    for var, name in ((1, "sweep_f32_r12_gauge"), (2, "gauge_sweep_f32"), (3, "gauge_sweep_f64arith")):
+    for var, name in ((1, "sweep_f32_r12_gauge"), (2, "gauge_sweep_f32"), (3, "gauge_sweep_f64arith"),
+                      (4, "gauge_sweep_q24")):
         fresh32()
         kms[name] = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var)
     reset()
@@ -411,17 +412,20 @@ def main():
     #   link sweeps: 16 R sizeof(real), R = 12 (rows 0,1 of every link read and written once);
     #   FP32 links + g: also the site's 12-float g read and written (+96 B);
     #   gauge-transformation sweep: U0 read once (8 x 48 B), g of the other parity read once
-    #   (48 B), the site's g read and written (96 B): 528 B
+    #   (48 B), the site's g read and written (96 B): 528 B; with U0 in 24-bit fixed
+    #   point (8 x 36 B): 432 B
     kbytes = {
         "sweep_f32_r12_gauge": (V // 2) * (16 * 12 * 4 + 2 * 12 * 4),
         "gauge_sweep_f32": (V // 2) * 528,
         "gauge_sweep_f64arith": (V // 2) * 528,
+        "gauge_sweep_q24": (V // 2) * 432,
         "sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
         "dropin_sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
     }
     t32 = r.ms_fp32 / max(r.sweeps_fp32, 1)
     t64 = (r.ms_total - r.ms_fp32) / max(r.sweeps - r.sweeps_fp32, 1)
-    k32name = {0: "sweep_f32_r12_gauge", 1: "gauge_sweep_f32", 2: "gauge_sweep_f64arith"}[args.mode]
+    k32name = {0: "sweep_f32_r12_gauge", 1: "gauge_sweep_f32", 2: "gauge_sweep_f64arith",
+               3: "gauge_sweep_q24"}[args.mode]
     dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms["sweep_f64_pattern18"]
     kname = k32name if dom32 else "sweep_f64_pattern18"
     achieved = kbytes[kname] / (kms[kname] / 1e3) / 1e9

@@ -53,7 +53,8 @@ typedef enum { GF_CRIT_MAX = 0, GF_CRIT_AVERAGE = 1 } gf_crit;
 typedef enum {
     GF_MIXED_LINKS = 0,   /* FP32 links updated together with an FP32 g(x) (any gauge) */
     GF_MIXED_GAUGE32 = 1, /* g(x) swept on read-only FP32 links, FP32 arithmetic */
-    GF_MIXED_GAUGE64 = 2  /* g(x) swept on read-only FP32 links, FP64 arithmetic */
+    GF_MIXED_GAUGE64 = 2, /* g(x) swept on read-only FP32 links, FP64 arithmetic */
+    GF_MIXED_GAUGE24 = 3  /* g(x) swept on read-only links in 24-bit fixed point, FP32 arithmetic */
 } gf_mixed_mode;
 
 typedef struct {
@@ -219,7 +220,8 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
  * only (lat = FP32, 12-real layout), without the check (the FP32 phases check
  * every check32 sweeps): 1 FP32 links with gauge accumulation, 2 / 3 the
  * gauge-transformation sweep with FP32 / FP64 arithmetic (U is then the
- * read-only link copy).  U is modified by variants 0 and 1. */
+ * read-only link copy), 4 the same as 2 on the links packed into 24-bit
+ * fixed point.  U is modified by variants 0 and 1. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
 
@@ -240,6 +242,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  *     in FP32 or FP64 arithmetic; Landau, Coulomb (4D lattice: every
  *     timeslice at once) and U1xU1.  refresh and reproject are not used in
  *     this phase.
+ *   mode GF_MIXED_GAUGE24: as GAUGE32 with U0 held in 24-bit fixed point
+ *     (each real as round(v (2^23 - 1)), four to three 32-bit words): 36
+ *     instead of 48 bytes per link, a quantisation error of at most 2^-24
+ *     per real, the size of an FP32 rounding error of an entry near 1.
  * No reference counterpart (the reference's DPUPDATES is a compile-time
  * FP32/FP64 mix, gaugefixing/apps/Makefile:46-48). */
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,

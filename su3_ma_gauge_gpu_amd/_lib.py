@@ -15,7 +15,7 @@ LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
-MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64 = 0, 1, 2
+MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24 = 0, 1, 2, 3
 CRIT_MAX, CRIT_AVERAGE = 0, 1
 
 _STATUS_NAMES = {

@@ -801,16 +801,16 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
-    if (variant < 0 || variant > 3) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
+    if (variant < 0 || variant > 4) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
     if (gauge == GF_COULOMB && lat->dims[0] == 1) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D lattices");
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
-    if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-3 time FP32 kernels");
+    if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
     if (variant >= 2 && gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
     hipStream_t s = S(stream);
     FusedCtx fc;
     GF_TRY(fc.init(V / 2, s));
-    Scratch gbuf;
+    Scratch gbuf, pbuf;
     struct {
         float *p = nullptr;
     } gb;
@@ -818,6 +818,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         HIP_TRY(gbuf.alloc(sizeof(float) * 12 * V, s));
         gb.p = gbuf.as<float>();
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
+    }
+    if (variant == 4) {  // the links in the 24-bit fixed-point form
+        HIP_TRY(pbuf.alloc(sizeof(uint32_t) * 36 * V, s));
+        HIP_TRY(launch_pack24(GF_F32, 12, U, pbuf.as<uint32_t>(), V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
     HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, s));
@@ -842,8 +846,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
                                                       s));
                 } else {
                     GaugeSweepArgs a{};
-                    a.U0 = (const float *)U; a.g = gb.p; a.state = fc.st; a.partial = partial; a.geo = geo;
-                    a.parity = par; a.param = 1.7f; a.remap = 1;
+                    const bool q24 = variant == 4;
+                    a.U0 = q24 ? pbuf.as<const float>() : (const float *)U;
+                    a.g = gb.p; a.state = fc.st; a.partial = partial; a.geo = geo;
+                    a.parity = par; a.param = 1.7f; a.remap = 1; a.links24 = q24;
                     HIP_TRY(launch_gsweep(gauge, GF_OR, 0, variant == 3, a, s));
                 }
             }
@@ -873,7 +879,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
     if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "the mixed driver takes an FP64 configuration");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
-    if (mode < GF_MIXED_LINKS || mode > GF_MIXED_GAUGE64) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
+    if (mode < GF_MIXED_LINKS || mode > GF_MIXED_GAUGE24) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
     if (gauge == GF_COULOMB && lat->dims[0] == 1)
         return fail(GF_ERR_BAD_ARG, "gf_run_mixed fixes every timeslice of a 4D lattice at once");
     if (gauge == GF_COULOMB && mode == GF_MIXED_LINKS)
@@ -906,7 +912,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     float *g = (float *)b.g;
     HIP_TRY(hipEventRecord(ev.a, s));
     HIP_TRY(launch_gauge_identity(g, V, s));
-    HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
+    if (mode == GF_MIXED_GAUGE24) HIP_TRY(launch_pack24(GF_F64, lat->layout, U, (uint32_t *)b.U32, V, s));
+    else HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
     int done32 = 0;
     if (mode == GF_MIXED_LINKS) {
         // FP32 links and gauge transformation; every `refresh` sweeps the
@@ -935,6 +942,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         ga.geo = geo;
         ga.param = omega;
         ga.remap = 1;
+        ga.links24 = mode == GF_MIXED_GAUGE24;
         auto launch = [&](int parity, int theta, double *partial, int) {
             ga.parity = parity;
             ga.partial = partial;

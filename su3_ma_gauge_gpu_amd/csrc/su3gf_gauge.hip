@@ -103,11 +103,11 @@ struct Field {
     __amdgpu_buffer_rsrc_t r;
     uint32_t V4;  // bytes per component plane
 };
-__device__ __forceinline__ Field make_field(const float *base, int64_t V)
+__device__ __forceinline__ Field make_field(const float *base, int64_t V, int words = 12)
 {
     Field f;
     // word 3: the untyped-access format the gfx9-family buffer instructions expect
-    f.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(48 * V), 0x00020000);
+    f.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(4 * words * V), 0x00020000);
     f.V4 = (uint32_t)(4 * V);
     return f;
 }
@@ -126,6 +126,48 @@ template <typename A> __device__ __forceinline__ void load_rows01(const Field &f
                 float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((comp + 1) * f.V4), 0));
             u.e[i][j] = cx((A)re, (A)im);
         }
+}
+
+// Links in 24-bit fixed point: each of the 12 reals of rows 0,1 is stored as
+// round(v * kQ24), a 24-bit two's-complement integer (|v| <= 1, so the
+// quantisation error is at most 2^-24, the size of an FP32 rounding error of
+// an entry near 1), and four of them share three 32-bit words:
+//   w0 = a | b << 24,   w1 = b >> 8 | c << 16,   w2 = c >> 16 | d << 8.
+// Nine component planes per link instead of twelve: 36 B instead of 48.  The
+// loads scale by 1/kQ24 at once: row 2 is rebuilt from rows 0,1 and is
+// quadratic in them, so unscaled integers would not form an SU(3) matrix.
+constexpr double kQ24 = 8388607.0;  // 2^23 - 1: +-1 are exact
+constexpr float kInvQ24 = (float)(1.0 / kQ24);
+
+template <typename A> __device__ __forceinline__ void load_rows01_q24(const Field &f, uint32_t boff, M3<A> &u)
+{
+    A v[12];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t w0 = __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((3 * k) * f.V4), 0);
+        const uint32_t w1 = __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((3 * k + 1) * f.V4), 0);
+        const uint32_t w2 = __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((3 * k + 2) * f.V4), 0);
+        // the bit-field builtin's result type is unsigned: convert it as the signed value it holds
+        const int a = (int)__builtin_amdgcn_sbfe((int)w0, 0, 24);
+        const int b = (int)__builtin_amdgcn_sbfe((int)__builtin_amdgcn_alignbit(w1, w0, 24), 0, 24);
+        const int c = (int)__builtin_amdgcn_sbfe((int)__builtin_amdgcn_alignbit(w2, w1, 16), 0, 24);
+        const int d = (int)w2 >> 8;
+        v[4 * k + 0] = (A)a * (A)kInvQ24;
+        v[4 * k + 1] = (A)b * (A)kInvQ24;
+        v[4 * k + 2] = (A)c * (A)kInvQ24;
+        v[4 * k + 3] = (A)d * (A)kInvQ24;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) u.e[i][j] = cx(v[2 * (j + 3 * i)], v[2 * (j + 3 * i) + 1]);
+}
+
+// U0 rows 0,1 in FP32 (LF = 0) or 24-bit fixed point (LF = 1)
+template <typename A, int LF> __device__ __forceinline__ void load_links(const Field &f, uint32_t boff, M3<A> &u)
+{
+    if (LF) load_rows01_q24(f, boff, u);
+    else load_rows01(f, boff, u);
 }
 
 template <typename A> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
@@ -326,104 +368,41 @@ __device__ __forceinline__ void gauge_update(A q[4], float param, Philox &rng)
     }
 }
 
-// Work mapping: a block of 256 threads covers 64 sites as four waves, wave mu
-// forming the direction-mu terms g(y) U0_mu(x)^dagger and g(z) U0_mu(z) of its
-// 64 sites (four 12-real loads per lane, all coalesced).  The terms go through
-// LDS; wave 0 adds them in the order mu = 0..3, builds M, runs the three
-// subgroup updates on it and stores g(x).
-template <typename A, int G, int ALGO, int THETA>
-__global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
+// The direction-mu terms of site x (coordinates c, parity par):
+// X = U0_mu(x) g(y)^dagger and Z = g(z) U0_mu(z), y = x+mu, z = x-mu, so that
+// S_mu = X^dagger + Z (Coulomb: zero for mu = 0, the time direction).
+template <typename A, int G, int LF>
+__device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const Field &fg, int mu, const int c[4],
+                                                int par, uint32_t site_off, M3<A> &X, M3<A> &Z)
 {
-    constexpr int kMats = THETA ? 2 : 1;  // P and Q apart for theta, else their sum
-    __shared__ A lds[kMats * 18 * 4 * 64];
-    if (a.state && a.state->done) return;  // converged: uniform early exit
-    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
-    const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
-    const int64_t p = blk * 64 + lane;
-    const bool active = p < a.geo.Vh;
-    const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
+    if (G == GF_COULOMB && mu == 0) {
+        zero(X);
+        zero(Z);
+        return;
+    }
     const int64_t V = a.geo.V;
-    int c[4], par;
-    site_coords(a.geo, site, c, par);
-    const uint32_t site_off = (uint32_t)site * 4u;
-    const Field fg = make_field(a.g, V);
+    M3<A> T, W;
+    const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
+    const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
+    const Field fu = LF ? make_field(a.U0 + 9 * V * mu, V, 9) : make_field(a.U0 + 12 * V * mu, V);
+    load_links<A, LF>(fu, site_off, W);
+    load_rows01(fg, y_off, T);
+    third_row(T);
+    prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
+    load_rows01(fg, z_off, T);
+    load_links<A, LF>(fu, z_off, W);
+    third_row(W);
+    prod_ab(Z, T, W);  // g(z) U0_mu(z)
+}
 
-    M3<A> Gx;
-    if (mu == 0) load_rows01(fg, site_off, Gx);
-
-    Philox rng;
-    if (ALGO == GF_SA || ALGO == GF_RANDOM) {
-        // key word 0 = global id of the reference's (mu = 0, up) thread of this
-        // site in its V/2/NSB x 8*NSB launch (PhiloxWrapper.hxx:93-110)
-        const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
-        philox_init(rng, tid, a.seed, a.counter);
-    }
-
-    if (ALGO == GF_RANDOM) {
-        // the random transformation does not depend on the links
-        if (mu != 0) return;
-    } else {
-        // S_mu = (U0_mu(x) g(y)^dagger)^dagger + g(z) U0_mu(z)
-        M3<A> X, Z;
-        if (!(G == GF_COULOMB && mu == 0)) {
-            M3<A> T, W;
-            const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
-            const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
-            const Field fu = make_field(a.U0 + 12 * V * mu, V);
-            load_rows01(fu, site_off, W);
-            load_rows01(fg, y_off, T);
-            third_row(T);
-            prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
-            load_rows01(fg, z_off, T);
-            load_rows01(fu, z_off, W);
-            third_row(W);
-            prod_ab(Z, T, W);  // g(z) U0_mu(z)
-        } else {
-            zero(X);
-            zero(Z);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int k = 2 * (3 * i + j);
-                if (THETA) {
-                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x;
-                    lds[((k + 1) * 4 + mu) * 64 + lane] = -X.e[j][i].y;
-                    lds[((18 + k + 0) * 4 + mu) * 64 + lane] = Z.e[i][j].x;
-                    lds[((18 + k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y;
-                } else {
-                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
-                    lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
-                }
-            }
-        __syncthreads();
-        if (mu != 0) return;
-    }
-
-    // P = sum_mu g(y) U0_mu(x)^dagger, Q = sum_mu g(z) U0_mu(z) (theta), or S = P + Q
-    M3<A> P, Q;
-    if (ALGO != GF_RANDOM) {
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int k = 2 * (3 * i + j);
-                A v[2], w[2];
-#pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    const A *b = lds + (k + r) * 4 * 64 + lane;
-                    v[r] = ((b[0] + b[64]) + b[128]) + b[192];
-                    if (THETA) {
-                        const A *bq = lds + (18 + k + r) * 4 * 64 + lane;
-                        w[r] = ((bq[0] + bq[64]) + bq[128]) + bq[192];
-                    }
-                }
-                P.e[i][j] = cx(v[0], v[1]);
-                if (THETA) Q.e[i][j] = cx(w[0], w[1]);
-            }
-    }
+// Given P = sum_mu g(y) U0_mu(x)^dagger and Q = sum_mu g(z) U0_mu(z) (THETA),
+// or their sum in P, and g(x) rows 0,1 in Gx: the three subgroup updates of
+// g(x), its store, and the fused-check partial sums of the 64-site block blk.
+template <typename A, int G, int ALGO, int THETA>
+__device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &fg, uint32_t site_off, bool active,
+                                          int64_t blk, int lane, const M3<A> &P, const M3<A> &Q, M3<A> Gx,
+                                          Philox &rng)
+{
     third_row(Gx);
 
     double th = 0, gf = 0;
@@ -483,25 +462,152 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     }
 }
 
-template <typename A, int G> hipError_t launch_g(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
+// Philox generator of a site (SA, random transformation): key word 0 = global
+// id of the reference's (mu = 0, up) thread of this site in its V/2/NSB x
+// 8*NSB launch (PhiloxWrapper.hxx:93-110)
+__device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, Philox &rng)
+{
+    const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
+    philox_init(rng, tid, a.seed, a.counter);
+}
+
+// Work mapping 0: a block of 256 threads covers 64 sites as four waves, wave mu
+// forming the direction-mu terms of its 64 sites (four 12-real loads per
+// lane, all coalesced).  The terms go through LDS; wave 0 adds them in the
+// order mu = 0..3 and runs the tail.
+template <typename A, int G, int ALGO, int THETA, int LF>
+__global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
+{
+    constexpr int kMats = THETA ? 2 : 1;  // P and Q apart for theta, else their sum
+    __shared__ A lds[kMats * 18 * 4 * 64];
+    if (a.state && a.state->done) return;  // converged: uniform early exit
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
+    const int64_t p = blk * 64 + lane;
+    const bool active = p < a.geo.Vh;
+    const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
+    int c[4], par;
+    site_coords(a.geo, site, c, par);
+    const uint32_t site_off = (uint32_t)site * 4u;
+    const Field fg = make_field(a.g, a.geo.V);
+
+    M3<A> Gx;
+    if (mu == 0) load_rows01(fg, site_off, Gx);
+
+    Philox rng;
+    if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, rng);
+
+    if (ALGO == GF_RANDOM) {
+        // the random transformation does not depend on the links
+        if (mu != 0) return;
+    } else {
+        M3<A> X, Z;
+        direction_terms<A, G, LF>(a, fg, mu, c, par, site_off, X, Z);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 2 * (3 * i + j);
+                if (THETA) {
+                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x;
+                    lds[((k + 1) * 4 + mu) * 64 + lane] = -X.e[j][i].y;
+                    lds[((18 + k + 0) * 4 + mu) * 64 + lane] = Z.e[i][j].x;
+                    lds[((18 + k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y;
+                } else {
+                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
+                    lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
+                }
+            }
+        __syncthreads();
+        if (mu != 0) return;
+    }
+
+    // P = sum_mu g(y) U0_mu(x)^dagger, Q = sum_mu g(z) U0_mu(z) (theta), or S = P + Q
+    M3<A> P, Q;
+    if (ALGO != GF_RANDOM) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 2 * (3 * i + j);
+                A v[2], w[2];
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const A *b = lds + (k + r) * 4 * 64 + lane;
+                    v[r] = ((b[0] + b[64]) + b[128]) + b[192];
+                    if (THETA) {
+                        const A *bq = lds + (18 + k + r) * 4 * 64 + lane;
+                        w[r] = ((bq[0] + bq[64]) + bq[128]) + bq[192];
+                    }
+                }
+                P.e[i][j] = cx(v[0], v[1]);
+                if (THETA) Q.e[i][j] = cx(w[0], w[1]);
+            }
+    }
+    site_tail<A, G, ALGO, THETA>(a, fg, site_off, active, blk, lane, P, Q, Gx, rng);
+}
+
+template <typename A, int G, int ALGO, int THETA, int LF>
+void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 {
     const dim3 grid((unsigned)gsweep_blocks(a.geo.Vh)), block(kGBlock);
-    if (algo == GF_OR) {
-        if (theta == 1) hipLaunchKernelGGL((k_gsweep<A, G, GF_OR, 1>), grid, block, 0, s, a);
-        else if (theta == 2) hipLaunchKernelGGL((k_gsweep<A, G, GF_OR, 2>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((k_gsweep<A, G, GF_OR, 0>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_gsweep<A, G, ALGO, THETA, LF>), grid, block, 0, s, a);
+}
+
+template <typename A, int G> hipError_t launch_g(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
+{
+    if (a.links24) {  // fixed-point links: the OR sweep of the mixed driver
+        if (algo != GF_OR) return hipErrorInvalidValue;
+        if (theta == 1) launch_k<A, G, GF_OR, 1, 1>(a, s);
+        else if (theta == 2) launch_k<A, G, GF_OR, 2, 1>(a, s);
+        else launch_k<A, G, GF_OR, 0, 1>(a, s);
+    } else if (algo == GF_OR) {
+        if (theta == 1) launch_k<A, G, GF_OR, 1, 0>(a, s);
+        else if (theta == 2) launch_k<A, G, GF_OR, 2, 0>(a, s);
+        else launch_k<A, G, GF_OR, 0, 0>(a, s);
     } else if (theta != 0) {
         return hipErrorInvalidValue;
     } else if (algo == GF_MICRO) {
-        hipLaunchKernelGGL((k_gsweep<A, G, GF_MICRO, 0>), grid, block, 0, s, a);
+        launch_k<A, G, GF_MICRO, 0, 0>(a, s);
     } else if (algo == GF_SA) {
-        hipLaunchKernelGGL((k_gsweep<A, G, GF_SA, 0>), grid, block, 0, s, a);
+        launch_k<A, G, GF_SA, 0, 0>(a, s);
     } else if (algo == GF_RANDOM) {
-        hipLaunchKernelGGL((k_gsweep<A, G, GF_RANDOM, 0>), grid, block, 0, s, a);
+        launch_k<A, G, GF_RANDOM, 0, 0>(a, s);
     } else {
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// links (type R, layout SI) -> rows 0,1 in the 24-bit fixed-point form, nine
+// component planes per direction: word w of link (i, mu) at P[(9 mu + w) V + i]
+template <typename R, int SI> __global__ __launch_bounds__(256) void k_pack24(const R *U, uint32_t *P, int64_t V)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 4 * V) return;
+    const int mu = (int)(t / V);
+    const int64_t i = t - (int64_t)mu * V;
+    const LinkRef<R> l = link_ref(const_cast<R *>(U), V, SI, i, mu);
+    M3<R> u;
+    load12(l, u);
+    uint32_t q[12];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int k = 2 * (c + 3 * r);
+            q[k] = (uint32_t)(int)rint(fmin(fmax((double)u.e[r][c].x, -1.0), 1.0) * kQ24) & 0xFFFFFFu;
+            q[k + 1] = (uint32_t)(int)rint(fmin(fmax((double)u.e[r][c].y, -1.0), 1.0) * kQ24) & 0xFFFFFFu;
+        }
+    uint32_t *o = P + (int64_t)9 * V * mu + i;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t a = q[4 * k], b = q[4 * k + 1], c = q[4 * k + 2], d = q[4 * k + 3];
+        o[(3 * k) * V] = a | (b << 24);
+        o[(3 * k + 1) * V] = (b >> 8) | (c << 16);
+        o[(3 * k + 2) * V] = (c >> 16) | (d << 8);
+    }
 }
 
 template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
@@ -515,6 +621,19 @@ template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const 
 }  // namespace
 
 int64_t gsweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
+
+hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s)
+{
+    if (36 * V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((4 * V + 255) / 256)), block(256);
+    const double *d = (const double *)U;
+    const float *f = (const float *)U;
+    if (ti == GF_F64 && si == 18) hipLaunchKernelGGL((k_pack24<double, 18>), grid, block, 0, s, d, P, V);
+    else if (ti == GF_F64 && si == 12) hipLaunchKernelGGL((k_pack24<double, 12>), grid, block, 0, s, d, P, V);
+    else if (ti == GF_F32 && si == 12) hipLaunchKernelGGL((k_pack24<float, 12>), grid, block, 0, s, f, P, V);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
 {

@@ -53,7 +53,7 @@ inline int64_t sweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 // Single-precision sweep on the gauge transformation (su3gf_gauge.hip): U0 is
 // read only, g(x) is updated; the links are g(x) U0_mu(x) g(x+mu)^dagger.
 struct GaugeSweepArgs {
-    const float *U0;   // compact FP32 links (rows 0,1, stride 12)
+    const float *U0;   // compact FP32 links (rows 0,1, stride 12), or the 24-bit words (links24)
     float *g;          // gauge transformation, rows 0,1 (stride 12, one field)
     DevState *state;   // fused-theta state (or null)
     double *partial;   // per-block (theta, gff) partials (fused check)
@@ -65,12 +65,16 @@ struct GaugeSweepArgs {
     int64_t slice_blocks;
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
+    int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR only)
 };
 int64_t gsweep_blocks(int64_t Vh);
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
 // 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
 // compute in FP64 on the FP32 data
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+// links of dtype ti and layout si (FP64 18 or 12, FP32 12) -> the 24-bit
+// fixed-point rows 0,1 (9 words per link)
+hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s);
 
 // exact translation unit (reference order, no FMA contraction)
 namespace exact {

@@ -4,8 +4,8 @@ for kernel traces and hardware-counter runs:
 * one device-to-device copy of the FP64 18-real configuration (a plain copy
   of known size, 4 * V * 18 * 8 bytes read and written), three times;
 * `--sweeps` FP32 gauge-transformation sweeps on read-only compact FP32
-  links, FP32 arithmetic (the mixed driver's default FP32 kernel), and the
-  same with FP64 arithmetic;
+  links, FP32 arithmetic (the mixed driver's default FP32 kernel), the
+  same with FP64 arithmetic, and on the links in 24-bit fixed point;
 * `--sweeps` FP32 link sweeps with gauge accumulation on the compact FP32
   layout (the mixed driver's link mode);
 * `--sweeps` fused-check FP64 sweeps on the 18-real layout (the FP64 kernel);
@@ -45,6 +45,7 @@ def main() -> None:
               None)
     ms = C.c_float()
     for variant, what in ((2, "fp32 gauge-transformation sweep"), (3, "fp32 gauge-transformation sweep, fp64 arithmetic"),
+                          (4, "fp32 gauge-transformation sweep, 24-bit fixed-point links"),
                           (1, "fp32 link sweep with gauge accumulation")):
         _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, variant,
                   C.byref(ms), None)

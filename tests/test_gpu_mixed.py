@@ -20,7 +20,7 @@ def _cfg(gf, dims, sigma=0.3, seed=1, haar=2):
     return gf.make_config(dims, sigma, seed, haar)
 
 
-@pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6)])
+@pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6), (3, 2e-5)])
 @pytest.mark.parametrize("gauge", [LANDAU, U1XU1])
 def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
     # U1xU1 normalises (q0, q3) alone, which magnifies the FP32 storage
@@ -67,7 +67,7 @@ def _oracle_coulomb_slices(oracle, dims, U4, precision=1e-12, maxiter=20000):
     return out
 
 
-@pytest.mark.parametrize("driver", ["fast", "mixed"])
+@pytest.mark.parametrize("driver", ["fast", "mixed", "mixed24"])
 def test_coulomb_all_slices_at_once(gf, gpu, oracle, driver):
     """Coulomb gauge on an 8^3 x 8 lattice, all timeslices in one run (each
     slice must meet theta < 1e-12), against the oracle fixing slice after
@@ -81,7 +81,9 @@ def test_coulomb_all_slices_at_once(gf, gpu, oracle, driver):
     if driver == "fast":
         res = gf.run_fast(lat, d, gf.COULOMB, 1.7, 1e-12, 20000, 100)
     else:
-        res = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-12, precision=1e-12, maxiter=20000)
+        mode = gf.MIXED_GAUGE24 if driver == "mixed24" else gf.MIXED_GAUGE32
+        res = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-12, precision=1e-12, maxiter=20000,
+                           mode=mode)
     assert res.converged == 1
     want = _oracle_coulomb_slices(oracle, dims, U)
     got = to_timeslices(d.download(), dims)

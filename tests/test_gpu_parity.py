@@ -257,12 +257,12 @@ def test_run_fast_fixed_sweeps(gf, gpu, oracle, gauge):
     assert res.theta == pytest.approx(t_end, rel=1e-6)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("dims,sigma,switch", [((8, 8, 8, 8), 0.3, 1e-6), ((12, 12, 12, 12), 0.25, 1e-6),
                                                ((8, 8, 8, 8), 0.3, 1e-12)])
 def test_run_mixed_matches_fp64_oracle(gf, gpu, oracle, dims, sigma, switch, mode):
-    """FP32 phase (links + g, or g alone in FP32 / FP64 arithmetic), then
-    FP64: the final functional is the FP64 oracle's to 1e-10 (same gauge
+    """FP32 phase (links + g, or g alone in FP32 / FP64 arithmetic, or on
+    24-bit fixed-point links), then FP64: the final functional is the FP64 oracle's to 1e-10 (same gauge
     orbit, same maximum)."""
     U = _config(gf, dims, sigma, 1, 2)
     d = gf.DeviceBuffer.from_numpy(U)
@@ -301,7 +301,7 @@ def _plaquette(M, dims):
     return tot / (6 * V * 3)
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_run_mixed_fp32_stagnation(gf, gpu, oracle, mode):
     """With a switch threshold FP32 cannot reach, the FP32 phase ends on
     stagnation and the FP64 phase still converges to the oracle's maximum."""

@@ -64,7 +64,8 @@ def parse():
     ap.add_argument("--mode", type=int, default=3, help="0 links+g, 1 g (FP32 arithmetic), 2 g (FP64 arithmetic), 3 g on 24-bit fixed-point links")
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--sa-steps", type=int, default=3000)
-    ap.add_argument("--sa-mode", type=int, default=1, help="0 FP64 link sweeps, 1 FP32 gauge-transformation sweeps")
+    ap.add_argument("--sa-mode", type=int, default=2,
+                    help="0 FP64 link sweeps, 1 FP32 gauge-transformation sweeps, 2 the same on 24-bit links")
     ap.add_argument("--mag-precision", type=float, default=1e-12)
     ap.add_argument("--workload", default="landau",
                     choices=["landau", "coulomb", "gribov", "mag"],

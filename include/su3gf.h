@@ -191,7 +191,8 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
  * every timeslice at once): the gauge transformation is annealed in FP32 on a
  * read-only FP32 copy of U (same update rules and random numbers, FP32
  * arithmetic as in the reference's single-precision build), then applied to U
- * in FP64 and U reprojected; returns when done. */
+ * in FP64 and U reprojected; returns when done.  mode 2: as mode 1 with the
+ * copy of U in 24-bit fixed point (see GF_MIXED_GAUGE24). */
 gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
                     int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
                     void *stream);

@@ -406,8 +406,9 @@ def run_sa(lattice: Lattice, U, gauge: int = LANDAU, tmax: float = 0.4, tmin: fl
            stream=None) -> tuple[int, int]:
     """Simulated annealing with the reference apps' schedule for ``gauge``
     (gf_run_sa in include/su3gf.h); mode 0 sweeps U itself, mode 1 anneals
-    the gauge transformation in FP32 and applies it to U.  Returns (next RNG
-    counter, number of temperatures visited)."""
+    the gauge transformation in FP32 and applies it to U, mode 2 does the
+    same on a copy of U in 24-bit fixed point.  Returns (next RNG counter,
+    number of temperatures visited)."""
     ctr = C.c_uint32(counter)
     ntemps = C.c_int()
     _lib.call("gf_run_sa", C.byref(lattice.struct), _ptr(U), _ptr(Udw), gauge, C.c_float(tmax), C.c_float(tmin),

@@ -517,27 +517,30 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     if (!U || !counter) return fail(GF_ERR_BAD_ARG, "null argument");
     if (steps < 0 || micro < 0) return fail(GF_ERR_BAD_ARG, "negative step count");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
-    if (mode != 0 && mode != 1) return fail(GF_ERR_BAD_ARG, "unknown annealing mode %d", mode);
-    if (mode == 1) {
-        if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "mode 1 takes an FP64 configuration");
+    if (mode < 0 || mode > 2) return fail(GF_ERR_BAD_ARG, "unknown annealing mode %d", mode);
+    const bool gform = mode != 0;  // the gauge transformation is annealed (modes 1, 2)
+    if (gform) {
+        if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "modes 1 and 2 take an FP64 configuration");
         if (gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
         if (gauge == GF_COULOMB && lat->dims[0] == 1)
-            return fail(GF_ERR_BAD_ARG, "mode 1 anneals every timeslice of a 4D lattice at once");
+            return fail(GF_ERR_BAD_ARG, "modes 1 and 2 anneal every timeslice of a 4D lattice at once");
     }
     hipStream_t s = S(stream);
     void *dw = gauge == GF_COULOMB ? Udw : U;
-    // mode 1: FP32 copy of U read only, the gauge transformation swept
+    // modes 1, 2: a read-only copy of U (FP32, or 24-bit fixed point), the gauge transformation swept
     gf_lattice l32 = *lat;
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
     Scratch bU32, bg;
     GaugeSweepArgs ga{};
     const Geom geo = make_geom(lat->dims, V);
-    if (mode == 1) {
+    if (gform) {
         HIP_TRY(bU32.alloc(gf_lattice_bytes(&l32), s));
         HIP_TRY(bg.alloc(sizeof(float) * 12 * V, s));
         HIP_TRY(launch_gauge_identity(bg.as<float>(), V, s));
-        HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, bU32.p, V, 0, s));
+        if (mode == 2) HIP_TRY(launch_pack24(lat->dtype, lat->layout, U, bU32.as<uint32_t>(), V, s));
+        else HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, bU32.p, V, 0, s));
+        ga.links24 = mode == 2;
         ga.U0 = bU32.as<const float>();
         ga.g = bg.as<float>();
         ga.geo = geo;
@@ -546,7 +549,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
         ga.nsb = 32;
     }
     auto project = [&]() -> gf_status {
-        if (mode == 1) return GF_OK;  // the gauge-transformation sweep keeps g unitary itself
+        if (gform) return GF_OK;  // the gauge-transformation sweep keeps g unitary itself
         GF_TRY(gf_project(lat, U, s));
         if (gauge == GF_COULOMB && Udw && Udw != U) GF_TRY(gf_project(lat, Udw, s));
         return GF_OK;
@@ -554,7 +557,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     // one heatbath temperature: SA on both parities, then `micro`
     // microcanonical pairs (LandauGaugeFixingSU3_4D.cu:214-223)
     auto heat = [&](float T) -> gf_status {
-        if (mode == 1) {
+        if (gform) {
             for (int par = 0; par < 2; par++) {
                 ga.parity = par;
                 ga.param = T;
@@ -601,7 +604,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
             n++;
         } while (temperature >= tmin);
     }
-    if (mode == 1) {
+    if (gform) {
         // U <- G U G^dagger in FP64 (G the FP64 projection of g), third rows rebuilt
         HIP_TRY(launch_project_gauge(bg.as<float>(), V, s));
         HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, bg.as<const float>(), U, GF_F64, lat->layout, geo,
@@ -671,9 +674,16 @@ int check_segments(const int dims[4], int gauge)
     return half_slice % 64 == 0 ? dims[0] : 1;
 }
 
+// coarse_every > check_every: while the last theta the host has read is above
+// kCoarseFactor * tol (or before the first check), check only every
+// coarse_every sweeps.  Such a theta cannot reach tol within the next chunk,
+// and stagnation is judged far below it, so only the cost of the checks
+// changes.  0 checks every check_every sweeps throughout.
+constexpr double kCoarseFactor = 1e3;
+
 template <typename Launch, typename After>
 gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int maxiter, int check_every,
-                     int stall_limit, int64_t nblocks, FusedCtx &fc, hipStream_t s, Launch launch,
+                     int coarse_every, int stall_limit, int64_t nblocks, FusedCtx &fc, hipStream_t s, Launch launch,
                      After after_sweep, int *sweeps_done, gf_run_result *r)
 {
     const int nseg = check_segments(dims, gauge);
@@ -684,11 +694,14 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
     if (e != hipSuccess) return hip_fail(e, "state init");
     int it = 0;
     bool done = false;
+    bool pending = false;  // a check of the state at the end of sweep it-1 is due
+    bool coarse = coarse_every > check_every;
     while (it < maxiter && !done) {
+        const int every = coarse ? coarse_every : check_every;
         const int n = std::min(kChunk, maxiter - it);
         for (int k = 0; k < n; k++, it++) {
-            const bool check_prev = it > 0 && it % check_every == 0;   // end of sweep it-1
-            const bool check_this = (it + 1) % check_every == 0;       // end of sweep it
+            const bool check_prev = pending;                    // end of sweep it-1
+            const bool check_this = (it + 1) % every == 0;      // end of sweep it
             e = launch(0, check_prev ? 1 : 0, fc.even(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
             if (check_prev) {
@@ -697,10 +710,12 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
             }
             e = launch(1, check_this ? 2 : 0, fc.odd(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
+            pending = check_this;
             GF_TRY(after_sweep(it));
         }
         GF_TRY(fc.read(s));
         done = fc.host->done != 0;
+        if (coarse && fc.host->checks > 0 && !(fc.host->theta > kCoarseFactor * tol)) coarse = false;
     }
     r->checks += fc.host->checks;
     if (done) {
@@ -718,8 +733,8 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
 // the fused loop over the link sweep kernels (FP64, or FP32 with g accumulation)
 template <typename R, typename After>
 gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
-                      int check_every, int stall_limit, FusedCtx &fc, hipStream_t s, After after_sweep,
-                      int *sweeps_done, gf_run_result *r)
+                      int check_every, int coarse_every, int stall_limit, FusedCtx &fc, hipStream_t s,
+                      After after_sweep, int *sweeps_done, gf_run_result *r)
 {
     int64_t V = 1;
     for (int i = 0; i < 4; i++) V *= lat->dims[i];
@@ -737,8 +752,8 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         a.partial = partial;
         return fast::launch_sweep<R>(lat->layout, gauge, GF_OR, theta, gacc != nullptr, a, s);
     };
-    return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, stall_limit, sweep_blocks(V / 2), fc, s,
-                      launch, after_sweep, sweeps_done, r);
+    return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit,
+                      sweep_blocks(V / 2), fc, s, launch, after_sweep, sweeps_done, r);
 }
 
 gf_status final_quality(const gf_lattice *lat, const void *U, int gauge, gf_run_result *r, hipStream_t s)
@@ -779,9 +794,9 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     };
     int done = 0;
     gf_status rc = lat->dtype == GF_F64
-                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, fc, s,
+                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, fc, s,
                                              after, &done, out)
-                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, fc, s,
+                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, fc, s,
                                             after, &done, out);
     GF_TRY(rc);
     out->sweeps = done;
@@ -874,6 +889,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     // FP32 checks without a 10% gain in theta (below stall_floor) before
     // switching anyway; MAG converges more slowly and gets a longer window
     const int kStall32 = gauge == GF_MAG ? 25 : 8;
+    // far above the switch threshold the FP32 phase checks 5x less often
+    const int kCoarse = 5;
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
@@ -930,7 +947,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
             }
             return GF_OK;
         };
-        GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32, kStall32,
+        GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32,
+                                  kCoarse * check32, kStall32,
                                   fc, s, after32, &done32, out));
     } else {
         // the links stay U0 (FP32 copy, read only); only g is swept, and the
@@ -949,7 +967,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
             return launch_gsweep(gauge, GF_OR, theta, mode == GF_MIXED_GAUGE64, ga, s);
         };
         auto none = [](int) -> gf_status { return GF_OK; };
-        GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kStall32, gsweep_blocks(V / 2),
+        GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kCoarse * check32, kStall32,
+                          gsweep_blocks(V / 2),
                           fc, s, launch, none, &done32, out));
     }
     out->sweeps_fp32 = done32;
@@ -964,7 +983,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     int done64 = 0;
     out->converged = 0;
     const int left = std::max(0, maxiter - done32);
-    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, 1, 0, fc, s, after64, &done64,
+    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, 1, 0, 0, fc, s, after64, &done64,
                                out));
     out->sweeps = done32 + done64;
     GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)

@@ -557,11 +557,20 @@ void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 
 template <typename A, int G> hipError_t launch_g(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
 {
-    if (a.links24) {  // fixed-point links: the OR sweep of the mixed driver
-        if (algo != GF_OR) return hipErrorInvalidValue;
-        if (theta == 1) launch_k<A, G, GF_OR, 1, 1>(a, s);
-        else if (theta == 2) launch_k<A, G, GF_OR, 2, 1>(a, s);
-        else launch_k<A, G, GF_OR, 0, 1>(a, s);
+    if (a.links24) {  // fixed-point links: OR of the mixed driver, SA and microcanonical of the annealing
+        if (algo == GF_OR) {
+            if (theta == 1) launch_k<A, G, GF_OR, 1, 1>(a, s);
+            else if (theta == 2) launch_k<A, G, GF_OR, 2, 1>(a, s);
+            else launch_k<A, G, GF_OR, 0, 1>(a, s);
+        } else if (theta != 0) {
+            return hipErrorInvalidValue;
+        } else if (algo == GF_MICRO) {
+            launch_k<A, G, GF_MICRO, 0, 1>(a, s);
+        } else if (algo == GF_SA) {
+            launch_k<A, G, GF_SA, 0, 1>(a, s);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else if (algo == GF_OR) {
         if (theta == 1) launch_k<A, G, GF_OR, 1, 0>(a, s);
         else if (theta == 2) launch_k<A, G, GF_OR, 2, 0>(a, s);

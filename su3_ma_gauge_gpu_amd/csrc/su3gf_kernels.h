@@ -65,7 +65,7 @@ struct GaugeSweepArgs {
     int64_t slice_blocks;
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
-    int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR only)
+    int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR, SA, microcanonical)
 };
 int64_t gsweep_blocks(int64_t Vh);
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,

@@ -130,8 +130,9 @@ def test_run_sa_schedule(gf, gpu, oracle, gauge, kw):
 
 
 def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
-    """Annealing the gauge transformation in FP32 (gf_run_sa mode 1) changes
-    the links only by a gauge transformation applied in FP64: the plaquette
+    """Annealing the gauge transformation in FP32 (gf_run_sa mode 1, and mode 2
+    on the links in 24-bit fixed point) changes the links only by a gauge
+    transformation applied in FP64: the plaquette
     is unchanged to 1e-13 and the links stay unitary; the functional rises
     like the reference-precision annealing's, and the OR loop then converges."""
     from test_gpu_parity import _plaquette
@@ -143,7 +144,7 @@ def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
     lat = gf.Lattice(dims)
     g0, _ = oracle.quality(LANDAU, dims, U.copy())
     res = {}
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         d = gf.DeviceBuffer.from_numpy(U)
         ctr, ntemps = gf.run_sa(lat, d, gf.LANDAU, 1.45, 0.01, 40, 3, 100, 1, 0, mode=mode)
         assert (ctr, ntemps) == (80, 40)
@@ -152,12 +153,13 @@ def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
         M = to_matrices(Ua, V)
         assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-13
         res[mode] = oracle.quality(LANDAU, dims, Ua.copy())[0]
-        if mode == 1:
+        if mode != 0:
             assert _plaquette(M, dims) == pytest.approx(_plaquette(to_matrices(U, V), dims), rel=1e-13)
             r = gf.run_or(lat, d, gf.LANDAU, 1.7, 1e-12, 5000, 20, 100)
             assert r.converged == 1
-    assert res[0] > g0 + 0.05 and res[1] > g0 + 0.05
+    assert res[0] > g0 + 0.05 and res[1] > g0 + 0.05 and res[2] > g0 + 0.05
     assert res[1] == pytest.approx(res[0], abs=0.02)
+    assert res[2] == pytest.approx(res[0], abs=0.02)
 
 
 def test_run_mixed_mag_links_mode(gf, gpu, oracle):

@@ -129,6 +129,20 @@ gf_status gf_set_hot(const gf_lattice *lat, void *U, int seed, uint32_t counter,
 gf_status gf_convert(const gf_lattice *src_lat, const void *src, const gf_lattice *dst_lat, void *dst,
                      int reunitarize, void *stream);
 
+/* The read-only link copy of GF_MIXED_GAUGE24 and gf_run_sa mode 2: rows 0,1
+ * of every link in 24-bit fixed point.  Each real v becomes
+ * q = round(v (2^23 - 1)) (v clamped to [-1, 1], ties to even), a 24-bit
+ * two's-complement integer; four of them, a b c d in the order
+ * Re u00, Im u00, Re u01, ... Im u12, share three words:
+ *   w0 = a | b << 24,   w1 = b >> 8 | c << 16,   w2 = c >> 16 | d << 8.
+ * Word w (0..8) of link (i, mu) is at P[(9 mu + w) V + i]; P holds 36 V words.
+ * gf_pack_links24 writes P from U (precision and layout of lat);
+ * gf_unpack_links24 decodes P the way the sweep kernels do, as
+ * (float)q * (float)(1 / (2^23 - 1)), into FP32 rows 0,1 in the compact layout
+ * (U32 holds 48 V floats).  No reference counterpart. */
+gf_status gf_pack_links24(const gf_lattice *lat, const void *U, uint32_t *P, void *stream);
+gf_status gf_unpack_links24(const int dims[4], const uint32_t *P, float *U32, void *stream);
+
 /* ------------------------------------------------------ quality / stats
  * Replaces GaugeFixingStats<Ndim,Nc,GType,StoppingCrit>
  * (gaugefixing/GaugeFixingStats.hxx:55-99,186-232): owns the reduction

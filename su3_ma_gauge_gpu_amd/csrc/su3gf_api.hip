@@ -352,6 +352,28 @@ gf_status gf_convert(const gf_lattice *src_lat, const void *src, const gf_lattic
     return GF_OK;
 }
 
+gf_status gf_pack_links24(const gf_lattice *lat, const void *U, uint32_t *P, void *stream)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!U || !P) return fail(GF_ERR_BAD_ARG, "null buffer");
+    if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
+        return fail(GF_ERR_BAD_ARG, "FP32 links are packed from the compact 12-real layout");
+    hipError_t e = launch_pack24(lat->dtype, lat->layout, U, P, V, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "pack kernel launch");
+    return GF_OK;
+}
+
+gf_status gf_unpack_links24(const int dims[4], const uint32_t *P, float *U32, void *stream)
+{
+    int64_t V;
+    GF_TRY(check_dims(dims, &V));
+    if (!P || !U32) return fail(GF_ERR_BAD_ARG, "null buffer");
+    hipError_t e = launch_unpack24(P, U32, V, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "unpack kernel launch");
+    return GF_OK;
+}
+
 // -------------------------------------------------------------- stats
 gf_status gf_stats_create(const int dims[4], int gauge, int crit, gf_stats **out)
 {

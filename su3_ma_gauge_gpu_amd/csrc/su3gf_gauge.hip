@@ -631,6 +631,36 @@ template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const 
 
 int64_t gsweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 
+namespace {
+// 24-bit links -> FP32 rows 0,1 (compact layout), through the sweep kernels' decode
+__global__ __launch_bounds__(256) void k_unpack24(const uint32_t *P, float *U32, int64_t V)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 4 * V) return;
+    const int mu = (int)(t / V);
+    const int64_t i = t - (int64_t)mu * V;
+    const Field f = make_field((const float *)P + 9 * V * mu, V, 9);
+    M3<float> u;
+    load_rows01_q24<float>(f, (uint32_t)i * 4u, u);
+    float *o = U32 + (int64_t)12 * V * mu + i;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            o[(2 * (c + 3 * r)) * V] = u.e[r][c].x;
+            o[(2 * (c + 3 * r) + 1) * V] = u.e[r][c].y;
+        }
+}
+}  // namespace
+
+hipError_t launch_unpack24(const uint32_t *P, float *U32, int64_t V, hipStream_t s)
+{
+    if (48 * V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((4 * V + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_unpack24, grid, block, 0, s, P, U32, V);
+    return hipGetLastError();
+}
+
 hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s)
 {
     if (36 * V >= (int64_t(1) << 31)) return hipErrorInvalidValue;

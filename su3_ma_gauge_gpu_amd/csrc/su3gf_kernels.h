@@ -75,6 +75,8 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
 // links of dtype ti and layout si (FP64 18 or 12, FP32 12) -> the 24-bit
 // fixed-point rows 0,1 (9 words per link)
 hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s);
+// the 24-bit rows 0,1 -> FP32 rows 0,1 (stride 12), decoded as the sweep kernels decode them
+hipError_t launch_unpack24(const uint32_t *P, float *U32, int64_t V, hipStream_t s);
 
 // exact translation unit (reference order, no FMA contraction)
 namespace exact {

@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--check32", type=int, default=5)
     ap.add_argument("--reproject", type=int, default=100)
     ap.add_argument("--maxiter", type=int, default=20000)
-    ap.add_argument("--kernel-sweeps", type=int, default=50)
+    ap.add_argument("--kernel-sweeps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

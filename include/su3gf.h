@@ -135,11 +135,15 @@ gf_status gf_convert(const gf_lattice *src_lat, const void *src, const gf_lattic
  * two's-complement integer; four of them, a b c d in the order
  * Re u00, Im u00, Re u01, ... Im u12, share three words:
  *   w0 = a | b << 24,   w1 = b >> 8 | c << 16,   w2 = c >> 16 | d << 8.
- * Word w (0..8) of link (i, mu) is at P[(9 mu + w) V + i]; P holds 36 V words.
+ * The words lie in tiles of 64 sites per direction: with T = ceil(V / 64),
+ * word w (0..8) of link (i, mu) is at
+ *   P[576 T mu + 9 (i - i % 64) + 64 w + i % 64],
+ * and P holds 2304 T words (gf_links24_words).
  * gf_pack_links24 writes P from U (precision and layout of lat);
  * gf_unpack_links24 decodes P the way the sweep kernels do, as
  * (float)q * (float)(1 / (2^23 - 1)), into FP32 rows 0,1 in the compact layout
  * (U32 holds 48 V floats).  No reference counterpart. */
+size_t gf_links24_words(const int dims[4]);
 gf_status gf_pack_links24(const gf_lattice *lat, const void *U, uint32_t *P, void *stream);
 gf_status gf_unpack_links24(const int dims[4], const uint32_t *P, float *U32, void *stream);
 
@@ -229,8 +233,8 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
                       int reproject, gf_run_result *out, void *stream);
 
 /* Average duration of one sweep kernel as the drivers launch it, from HIP
- * events around nsweeps back-to-back OR sweeps on `stream` (after two
- * warm-up sweeps).  variant 0: the link sweep in the precision of lat with
+ * events around nsweeps back-to-back OR sweeps on `stream` (after
+ * max(2, nsweeps / 2) warm-up sweeps).  variant 0: the link sweep in the precision of lat with
  * the fused check (gf_run_fast and the FP64 phase check every sweep); FP32
  * only (lat = FP32, 12-real layout), without the check (the FP32 phases check
  * every check32 sweeps): 1 FP32 links with gauge accumulation, 2 / 3 the

@@ -85,6 +85,7 @@ SIGNATURES = {
     "gf_project": (_i, [_pl, _vp, _vp]),
     "gf_set_hot": (_i, [_pl, _vp, _i, _u32, _vp]),
     "gf_convert": (_i, [_pl, _vp, _pl, _vp, _i, _vp]),
+    "gf_links24_words": (C.c_size_t, [_pi4]),
     "gf_pack_links24": (_i, [_pl, _vp, _vp, _vp]),
     "gf_unpack_links24": (_i, [_pi4, _vp, _vp, _vp]),
     "gf_stats_create": (_i, [_pi4, _i, _i, C.POINTER(_vp)]),

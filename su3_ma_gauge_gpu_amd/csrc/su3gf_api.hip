@@ -352,6 +352,13 @@ gf_status gf_convert(const gf_lattice *src_lat, const void *src, const gf_lattic
     return GF_OK;
 }
 
+size_t gf_links24_words(const int dims[4])
+{
+    int64_t V = 1;
+    for (int i = 0; i < 4; i++) V *= dims[i] > 0 ? dims[i] : 0;
+    return (size_t)links24_words(V);
+}
+
 gf_status gf_pack_links24(const gf_lattice *lat, const void *U, uint32_t *P, void *stream)
 {
     int64_t V;
@@ -557,8 +564,8 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     GaugeSweepArgs ga{};
     const Geom geo = make_geom(lat->dims, V);
     if (gform) {
-        HIP_TRY(bU32.alloc(gf_lattice_bytes(&l32), s));
-        HIP_TRY(bg.alloc(sizeof(float) * 12 * V, s));
+        HIP_TRY(bU32.alloc(mode == 2 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
+        HIP_TRY(bg.alloc(sizeof(float) * gauge_floats(V), s));
         HIP_TRY(launch_gauge_identity(bg.as<float>(), V, s));
         if (mode == 2) HIP_TRY(launch_pack24(lat->dtype, lat->layout, U, bU32.as<uint32_t>(), V, s));
         else HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, bU32.p, V, 0, s));
@@ -852,12 +859,12 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         float *p = nullptr;
     } gb;
     if (variant > 0) {
-        HIP_TRY(gbuf.alloc(sizeof(float) * 12 * V, s));
+        HIP_TRY(gbuf.alloc(sizeof(float) * gauge_floats(V), s));
         gb.p = gbuf.as<float>();
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
     if (variant == 4) {  // the links in the 24-bit fixed-point form
-        HIP_TRY(pbuf.alloc(sizeof(uint32_t) * 36 * V, s));
+        HIP_TRY(pbuf.alloc(sizeof(uint32_t) * links24_words(V), s));
         HIP_TRY(launch_pack24(GF_F32, 12, U, pbuf.as<uint32_t>(), V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
@@ -893,9 +900,12 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         }
         return GF_OK;
     };
-    GF_TRY(run(2, 0));  // warm-up
+    // warm-up: long enough for the clocks and caches to reach the steady state
+    // of a sustained run (a few sweeps alone read up to 20% slow)
+    const int warm = std::max(2, nsweeps / 2);
+    GF_TRY(run(warm, 0));
     HIP_TRY(hipEventRecord(ev.a, s));
-    GF_TRY(run(nsweeps, 2));
+    GF_TRY(run(nsweeps, warm));
     HIP_TRY(hipEventRecord(ev.b, s));
     float ms = 0;
     HIP_TRY(hipEventSynchronize(ev.b));
@@ -932,8 +942,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     l32.layout = GF_LAYOUT_R12;
     const Geom geo = make_geom(lat->dims, V);
     Scratch sU32, sg;
-    HIP_TRY(sU32.alloc(gf_lattice_bytes(&l32), s));
-    HIP_TRY(sg.alloc(sizeof(float) * 12 * V, s));
+    HIP_TRY(sU32.alloc(mode == GF_MIXED_GAUGE24 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
+    HIP_TRY(sg.alloc(sizeof(float) * gauge_floats(V), s));
     struct {
         void *U32, *g;
     } b{sU32.p, sg.p};

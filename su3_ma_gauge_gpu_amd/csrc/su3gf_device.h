@@ -128,6 +128,20 @@ template <typename R> __device__ __forceinline__ LinkRef<R> link_ref(R *base, in
     return l;
 }
 
+// The gauge transformation g of the mixed-precision and annealing drivers:
+// FP32 rows 0,1 per site in tiles of 64 sites, component c (0..11) of site i
+// at g[(i & ~63) * 12 + c * 64 + (i & 63)].  A block of 64 sites reads one
+// contiguous 3 KB tile instead of twelve planes V floats apart.  The buffer
+// holds gauge_floats(V) floats (whole tiles).
+__host__ __device__ inline int64_t gauge_floats(int64_t V) { return 12 * ((V + 63) / 64) * 64; }
+__device__ __forceinline__ LinkRef<float> gauge_ref(float *g, int64_t i)
+{
+    LinkRef<float> l;
+    l.p = g + (i & ~int64_t(63)) * 12 + (i & 63);
+    l.V = 64;  // component stride
+    return l;
+}
+
 template <typename R> __device__ __forceinline__ void load12(const LinkRef<R> &l, M3<R> &u)
 {
 #pragma unroll

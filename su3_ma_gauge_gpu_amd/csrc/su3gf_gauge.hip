@@ -112,6 +112,25 @@ __device__ __forceinline__ Field make_field(const float *base, int64_t V, int wo
     return f;
 }
 
+// A field in tiles of 64 sites, `words` 32-bit components per site: component
+// c of site i at word (i & ~63) * words + c * 64 + (i & 63) (the layout of
+// gauge_ref for g, words = 12, and of the 24-bit links, words = 9).  V4 is
+// then the component stride of a tile, 256 B, and a site's byte offset 4 i
+// becomes tile_off(4 i, words).
+__device__ __forceinline__ Field make_tiled_field(const float *base, int64_t V, int words)
+{
+    Field f;
+    const int64_t tiles = (V + 63) / 64;
+    f.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(4 * words * 64 * tiles),
+                                            0x00020000);
+    f.V4 = 256u;
+    return f;
+}
+__device__ __forceinline__ uint32_t tile_off(uint32_t boff, uint32_t words)
+{
+    return (boff & ~255u) * words + (boff & 255u);
+}
+
 // rows 0,1 of site byte offset boff
 template <typename A> __device__ __forceinline__ void load_rows01(const Field &f, uint32_t boff, M3<A> &u)
 {
@@ -133,7 +152,8 @@ template <typename A> __device__ __forceinline__ void load_rows01(const Field &f
 // quantisation error is at most 2^-24, the size of an FP32 rounding error of
 // an entry near 1), and four of them share three 32-bit words:
 //   w0 = a | b << 24,   w1 = b >> 8 | c << 16,   w2 = c >> 16 | d << 8.
-// Nine component planes per link instead of twelve: 36 B instead of 48.  The
+// Nine words per link instead of twelve: 36 B instead of 48, in tiles of 64
+// sites per direction (make_tiled_field, words = 9).  The
 // loads scale by 1/kQ24 at once: row 2 is rebuilt from rows 0,1 and is
 // quadratic in them, so unscaled integers would not form an SU(3) matrix.
 constexpr double kQ24 = 8388607.0;  // 2^23 - 1: +-1 are exact
@@ -163,11 +183,19 @@ template <typename A> __device__ __forceinline__ void load_rows01_q24(const Fiel
         for (int j = 0; j < 3; j++) u.e[i][j] = cx(v[2 * (j + 3 * i)], v[2 * (j + 3 * i) + 1]);
 }
 
-// U0 rows 0,1 in FP32 (LF = 0) or 24-bit fixed point (LF = 1)
+// U0 rows 0,1 of the site at byte offset 4 i: FP32 planes (LF = 0) or 24-bit
+// fixed point in tiles (LF = 1)
 template <typename A, int LF> __device__ __forceinline__ void load_links(const Field &f, uint32_t boff, M3<A> &u)
 {
-    if (LF) load_rows01_q24(f, boff, u);
+    if (LF) load_rows01_q24(f, tile_off(boff, 9), u);
     else load_rows01(f, boff, u);
+}
+
+// the field of U0_mu (one direction)
+template <int LF> __device__ __forceinline__ Field links_field(const float *U0, int64_t V, int mu)
+{
+    if (LF) return make_tiled_field(U0 + 9 * 64 * ((V + 63) / 64) * mu, V, 9);
+    return make_field(U0 + 12 * V * mu, V);
 }
 
 template <typename A> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
@@ -380,16 +408,15 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
         zero(Z);
         return;
     }
-    const int64_t V = a.geo.V;
     M3<A> T, W;
     const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
     const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
-    const Field fu = LF ? make_field(a.U0 + 9 * V * mu, V, 9) : make_field(a.U0 + 12 * V * mu, V);
+    const Field fu = links_field<LF>(a.U0, a.geo.V, mu);
     load_links<A, LF>(fu, site_off, W);
-    load_rows01(fg, y_off, T);
+    load_rows01(fg, tile_off(y_off, 12), T);
     third_row(T);
     prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
-    load_rows01(fg, z_off, T);
+    load_rows01(fg, tile_off(z_off, 12), T);
     load_links<A, LF>(fu, z_off, W);
     third_row(W);
     prod_ab(Z, T, W);  // g(z) U0_mu(z)
@@ -445,7 +472,7 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
     }
 
     project_rows01(Gx);  // keep g unitary at the precision it is stored in
-    if (active) store_rows01(fg, site_off, Gx);
+    if (active) store_rows01(fg, tile_off(site_off, 12), Gx);
 
     if (THETA == 1 && active) site_theta<G>(G0, P, Q, th, gf);
     if (THETA == 2 && active) {
@@ -490,10 +517,10 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     int c[4], par;
     site_coords(a.geo, site, c, par);
     const uint32_t site_off = (uint32_t)site * 4u;
-    const Field fg = make_field(a.g, a.geo.V);
+    const Field fg = make_tiled_field(a.g, a.geo.V, 12);
 
     M3<A> Gx;
-    if (mu == 0) load_rows01(fg, site_off, Gx);
+    if (mu == 0) load_rows01(fg, tile_off(site_off, 12), Gx);
 
     Philox rng;
     if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, rng);
@@ -589,8 +616,9 @@ template <typename A, int G> hipError_t launch_g(int algo, int theta, const Gaug
     return hipGetLastError();
 }
 
-// links (type R, layout SI) -> rows 0,1 in the 24-bit fixed-point form, nine
-// component planes per direction: word w of link (i, mu) at P[(9 mu + w) V + i]
+// links (type R, layout SI) -> rows 0,1 in the 24-bit fixed-point form, in
+// tiles of 64 sites per direction: word w of link (i, mu) at
+// P[mu links24_words(V) / 4 + (i & ~63) 9 + 64 w + (i & 63)]
 template <typename R, int SI> __global__ __launch_bounds__(256) void k_pack24(const R *U, uint32_t *P, int64_t V)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -609,13 +637,13 @@ template <typename R, int SI> __global__ __launch_bounds__(256) void k_pack24(co
             q[k] = (uint32_t)(int)rint(fmin(fmax((double)u.e[r][c].x, -1.0), 1.0) * kQ24) & 0xFFFFFFu;
             q[k + 1] = (uint32_t)(int)rint(fmin(fmax((double)u.e[r][c].y, -1.0), 1.0) * kQ24) & 0xFFFFFFu;
         }
-    uint32_t *o = P + (int64_t)9 * V * mu + i;
+    uint32_t *o = P + links24_words(V) / 4 * mu + (i & ~int64_t(63)) * 9 + (i & 63);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const uint32_t a = q[4 * k], b = q[4 * k + 1], c = q[4 * k + 2], d = q[4 * k + 3];
-        o[(3 * k) * V] = a | (b << 24);
-        o[(3 * k + 1) * V] = (b >> 8) | (c << 16);
-        o[(3 * k + 2) * V] = (c >> 16) | (d << 8);
+        o[(3 * k) * 64] = a | (b << 24);
+        o[(3 * k + 1) * 64] = (b >> 8) | (c << 16);
+        o[(3 * k + 2) * 64] = (c >> 16) | (d << 8);
     }
 }
 
@@ -639,9 +667,9 @@ __global__ __launch_bounds__(256) void k_unpack24(const uint32_t *P, float *U32,
     if (t >= 4 * V) return;
     const int mu = (int)(t / V);
     const int64_t i = t - (int64_t)mu * V;
-    const Field f = make_field((const float *)P + 9 * V * mu, V, 9);
+    const Field f = links_field<1>((const float *)P, V, mu);
     M3<float> u;
-    load_rows01_q24<float>(f, (uint32_t)i * 4u, u);
+    load_rows01_q24<float>(f, tile_off((uint32_t)i * 4u, 9), u);
     float *o = U32 + (int64_t)12 * V * mu + i;
 #pragma unroll
     for (int r = 0; r < 2; r++)
@@ -655,7 +683,7 @@ __global__ __launch_bounds__(256) void k_unpack24(const uint32_t *P, float *U32,
 
 hipError_t launch_unpack24(const uint32_t *P, float *U32, int64_t V, hipStream_t s)
 {
-    if (48 * V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    if (links24_words(V) >= (int64_t(1) << 31)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((4 * V + 255) / 256)), block(256);
     hipLaunchKernelGGL(k_unpack24, grid, block, 0, s, P, U32, V);
     return hipGetLastError();
@@ -663,7 +691,7 @@ hipError_t launch_unpack24(const uint32_t *P, float *U32, int64_t V, hipStream_t
 
 hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s)
 {
-    if (36 * V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    if (links24_words(V) >= (int64_t(1) << 31)) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((4 * V + 255) / 256)), block(256);
     const double *d = (const double *)U;
     const float *f = (const float *)U;
@@ -676,8 +704,8 @@ hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, 
 
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
 {
-    // 32-bit byte offsets: a buffer of 12 FP32 components per site must stay below 2^31 bytes
-    if (48 * a.geo.V >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    // 32-bit byte offsets: a buffer of 12 FP32 components per site (whole tiles) must stay below 2^31 bytes
+    if (4 * gauge_floats(a.geo.V) >= (int64_t(1) << 31)) return hipErrorInvalidValue;
     GaugeSweepArgs b = a;
     const int64_t half_slice = a.geo.Vh / a.geo.N[0];  // sites of one parity in a timeslice
     if (b.remap && a.geo.N[0] % 8 == 0 && half_slice % 64 == 0) {

@@ -72,6 +72,8 @@ int64_t gsweep_blocks(int64_t Vh);
 // 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
 // compute in FP64 on the FP32 data
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+// words of the 24-bit link copy: 9 per link, in whole tiles of 64 sites
+__host__ __device__ inline int64_t links24_words(int64_t V) { return 4 * 9 * 64 * ((V + 63) / 64); }
 // links of dtype ti and layout si (FP64 18 or 12, FP32 12) -> the 24-bit
 // fixed-point rows 0,1 (9 words per link)
 hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, hipStream_t s);

@@ -85,7 +85,7 @@ hipError_t launch_reduce_final(const double *partial, int64_t n, int crit_max, d
 }
 
 // ------------------------------------------------- gauge transformations
-// g: compact FP32 rows 0,1 per site, g[idx + V*comp]
+// g: FP32 rows 0,1 per site in tiles of 64 sites (gauge_ref)
 __global__ __launch_bounds__(kBlock) void k_gauge_identity(float *g, int64_t V)
 {
     const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_gauge_identity(float *g, int64_t V)
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) u.e[i][j] = cx((float)(i == j), 0.f);
-    store12(link_ref(g, V, 12, idx, 0), u);
+    store12(gauge_ref(g, idx), u);
 }
 
 hipError_t launch_gauge_identity(float *g, int64_t V, hipStream_t s)
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_project_gauge(float *g, int64_t V)
 {
     const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (idx >= V) return;
-    const LinkRef<float> l = link_ref(g, V, 12, idx, 0);
+    const LinkRef<float> l = gauge_ref(g, idx);
     M3<float> f;
     load12(l, f);
     M3<double> d;
@@ -131,10 +131,10 @@ hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s)
     return hipGetLastError();
 }
 
-__device__ __forceinline__ void gauge_at(const float *g, int64_t V, int64_t idx, M3<double> &G)
+__device__ __forceinline__ void gauge_at(const float *g, int64_t idx, M3<double> &G)
 {
     M3<float> f;
-    load12(link_ref(const_cast<float *>(g), V, 12, idx, 0), f);
+    load12(gauge_ref(const_cast<float *>(g), idx), f);
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -155,8 +155,8 @@ __global__ __launch_bounds__(kBlock) void k_gauge_apply(const double *U0, const 
     site_coords(geo, idx, c, par);
     const int64_t fw = forward(geo, c, par, mu);
     M3<double> Gx, Gf, U;
-    gauge_at(g, geo.V, idx, Gx);
-    gauge_at(g, geo.V, fw, Gf);
+    gauge_at(g, idx, Gx);
+    gauge_at(g, fw, Gf);
     load12(link_ref(const_cast<double *>(U0), geo.V, S0, idx, mu), U);
     reconstruct_e<false>(U);
     // T = Gx U ; M = T Gf^dagger (rows 0,1)

@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     LinkRef<float> lg;
     const bool own_g = GACC && mu == 0 && active;
     if (GACC) {
-        lg = link_ref(a.gacc, a.g.V, 12, site, 0);
+        lg = gauge_ref(a.gacc, site);
         if (own_g) {
             load12(lg, gt);
             reconstruct_e<false>(gt);

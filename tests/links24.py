@@ -33,20 +33,30 @@ def unpack_ints(w):
     return np.where(u >= (1 << 23), u - (1 << 24), u)
 
 
+def words(V):
+    """Words of the copy: 9 per link in whole tiles of 64 sites."""
+    return 4 * 9 * 64 * ((V + 63) // 64)
+
+
 def pack_config(U, V):
-    """FP64 links in the 18-real GpuPattern layout (V*72 reals) -> P (36 V words):
-    word w of link (i, mu) at P[(9 mu + w) V + i]."""
+    """FP64 links in the 18-real GpuPattern layout (V*72 reals) -> P: word w of
+    link (i, mu) at P[576 T mu + 9 (i - i % 64) + 64 w + i % 64], T = ceil(V / 64);
+    the padding of the last tile stays zero."""
     U = np.asarray(U, dtype=np.float64).reshape(4, 18, V)
     rows01 = U[:, :12, :]                       # (mu, component, site)
     q = quantise(rows01).transpose(0, 2, 1).reshape(4, V, 3, 4)
-    w = pack_words(q)                           # (mu, site, group, word)
-    return w.reshape(4, V, 9).transpose(0, 2, 1).reshape(-1)
+    w = pack_words(q).reshape(4, V, 9)          # (mu, site, word)
+    T = (V + 63) // 64
+    P = np.zeros((4, T * 64, 9), dtype=np.uint32)
+    P[:, :V, :] = w
+    return P.reshape(4, T, 64, 9).transpose(0, 1, 3, 2).reshape(-1)
 
 
 def unpack_config(P, V):
     """P -> FP32 rows 0,1 in the compact layout (48 V floats), decoded as
     (float)q * (float)(1 / (2^23 - 1))."""
-    w = np.asarray(P, dtype=np.uint32).reshape(4, 9, V).transpose(0, 2, 1).reshape(4, V, 3, 3)
-    q = unpack_ints(w).reshape(4, V, 12)
+    T = (V + 63) // 64
+    w = np.asarray(P, dtype=np.uint32).reshape(4, T, 9, 64).transpose(0, 1, 3, 2).reshape(4, T * 64, 9)[:, :V]
+    q = unpack_ints(w.reshape(4, V, 3, 3)).reshape(4, V, 12)
     v = q.astype(np.float32) * np.float32(1.0 / Q)
     return v.transpose(0, 2, 1).reshape(-1)

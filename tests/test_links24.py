@@ -5,7 +5,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from links24 import Q, pack_config, pack_words, quantise, unpack_config, unpack_ints
+from links24 import Q, pack_config, pack_words, quantise, unpack_config, unpack_ints, words
 
 
 def test_known_words():
@@ -34,19 +34,21 @@ def test_round_trip_error():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cold", [False, True])
-def test_device_pack_and_decode_bit_exact(gf, gpu, cold):
-    dims = (4, 6, 4, 8)
+@pytest.mark.parametrize("cold,dims", [(False, (4, 6, 4, 8)), (True, (4, 6, 4, 8)), (False, (6, 6, 6, 6))])
+def test_device_pack_and_decode_bit_exact(gf, gpu, cold, dims):
     V = int(np.prod(dims))
     U = gf.make_config(dims, 0.0 if cold else 0.4, 3, 0 if cold else 4)
     d = gf.DeviceBuffer.from_numpy(U)
     lat = gf.Lattice(dims)
-    P = gf.DeviceBuffer(36 * V * 4)
     from su3_ma_gauge_gpu_amd import _lib
+    n = _lib.load().gf_links24_words((C.c_int * 4)(*dims))
+    assert n == words(V)
+    P = gf.DeviceBuffer(4 * n)
+    _lib.call("gf_memset", C.c_void_p(P.ptr), 0, C.c_size_t(4 * n))
     _lib.call("gf_pack_links24", C.byref(lat.struct), C.c_void_p(d.ptr), C.c_void_p(P.ptr), None)
-    words = P.download(np.uint32)
+    got_words = P.download(np.uint32)
     want = pack_config(U, V)
-    assert np.array_equal(words, want)
+    assert np.array_equal(got_words, want)
     out = gf.DeviceBuffer(48 * V * 4)
     _lib.call("gf_unpack_links24", (C.c_int * 4)(*dims), C.c_void_p(P.ptr), C.c_void_p(out.ptr), None)
     got = out.download(np.float32)

@@ -33,7 +33,8 @@ typedef enum {
     GF_ERR_BAD_ARG = 2,       /* null pointer, unknown enum, bad parameter */
     GF_ERR_HIP = 3,           /* a HIP runtime call failed */
     GF_ERR_NOT_CONVERGED = 4, /* a driver hit its iteration limit */
-    GF_ERR_NO_DEVICE = 5      /* no usable gfx950 device */
+    GF_ERR_NO_DEVICE = 5,     /* no usable gfx950 device */
+    GF_ERR_IO = 6             /* a configuration file cannot be read or written (su3gf_io.h) */
 } gf_status;
 
 /* lattice/datatype/lattice_typedefs.h:39 (enum GaugeType) */

@@ -45,13 +45,23 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     SA,
     SR,
     U1XU1,
+    FILE_PLAIN,
+    FILE_HEADERONLY,
+    FILE_VOGT,
+    FILE_ILDG,
+    FILE_QCDSTAG,
+    REAL_STANDARD,
+    REAL_DOUBLE,
+    REAL_FLOAT,
+    PATTERN_GPU,
+    PATTERN_GPU_TIMESLICE,
     GFError,
 )
 
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "device_info", "RunResult",
+    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "device_info", "RunResult", "LinkFile",
 ]
 
 
@@ -433,6 +443,58 @@ def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, swit
               C.c_float(precision), int(maxiter), int(mode), int(refresh), int(check32), int(reproject), C.byref(r),
               _ptr(stream))
     return _result(r)
+
+
+# ------------------------------------------------------------ files
+class LinkFile:
+    """A configuration file of one type (include/su3gf_io.h), as the
+    reference's LinkFile<FileType, StandardPattern, GpuPattern, SiteCoord>
+    (lattice/LinkFile.hxx:60-72) and the apps' ILDG / QCDSTAG functions.
+    ``U`` is a host numpy array of all 18 reals per link (float64 or float32)
+    in GpuPattern order (or GpuPatternTimeslice with
+    ``pattern=PATTERN_GPU_TIMESLICE``).  The handle keeps the header or LIME
+    records of the last file loaded, for ``save``."""
+
+    def __init__(self, filetype: int, reinterpret: int = REAL_STANDARD, pattern: int = PATTERN_GPU):
+        h = C.c_void_p()
+        _lib.call("gf_linkfile_create", int(filetype), int(reinterpret), int(pattern), C.byref(h))
+        self._h = h
+        self.filetype = filetype
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().gf_linkfile_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _lattice(dims, U: np.ndarray) -> "Lattice":
+        dtype = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}.get(U.dtype)
+        if dtype is None:
+            raise TypeError("U must be float64 or float32")
+        return Lattice(tuple(dims), dtype, PATTERN18)
+
+    def load(self, dims, filename: str, U: np.ndarray) -> None:
+        """Fill ``U`` (contiguous, 72 V reals) from ``filename``."""
+        if not (U.flags.c_contiguous and U.flags.writeable):
+            raise ValueError("U must be a writeable contiguous array")
+        lat = self._lattice(dims, U)
+        if U.size != 72 * lat.V:
+            raise ValueError(f"U holds {U.size} reals, the lattice {72 * lat.V}")
+        _lib.call("gf_linkfile_load", self._h, str(filename).encode(), C.byref(lat.struct), U.ctypes.data_as(C.c_void_p))
+
+    def save(self, dims, filename: str, U: np.ndarray, sa_steps: int = 0) -> None:
+        U = np.ascontiguousarray(U)
+        lat = self._lattice(dims, U)
+        if U.size != 72 * lat.V:
+            raise ValueError(f"U holds {U.size} reals, the lattice {72 * lat.V}")
+        _lib.call("gf_linkfile_save", self._h, str(filename).encode(), C.byref(lat.struct),
+                  U.ctypes.data_as(C.c_void_p), int(sa_steps))
+
+    def read_header(self, dims, filename: str, dtype=np.float64) -> None:
+        """Keep the header / LIME records of ``filename`` without its links."""
+        lat = Lattice(tuple(dims), F64 if np.dtype(dtype) == np.float64 else F32, PATTERN18)
+        _lib.call("gf_linkfile_read_header", self._h, str(filename).encode(), C.byref(lat.struct))
 
 
 # --------------------------------------------------------- synthetic input

@@ -10,13 +10,16 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libsu3gf.so"
 
-GF_OK, GF_ERR_BAD_SHAPE, GF_ERR_BAD_ARG, GF_ERR_HIP, GF_ERR_NOT_CONVERGED, GF_ERR_NO_DEVICE = range(6)
+GF_OK, GF_ERR_BAD_SHAPE, GF_ERR_BAD_ARG, GF_ERR_HIP, GF_ERR_NOT_CONVERGED, GF_ERR_NO_DEVICE, GF_ERR_IO = range(7)
 LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
 MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24 = 0, 1, 2, 3
 CRIT_MAX, CRIT_AVERAGE = 0, 1
+FILE_PLAIN, FILE_HEADERONLY, FILE_VOGT, FILE_ILDG, FILE_QCDSTAG = range(5)
+REAL_STANDARD, REAL_DOUBLE, REAL_FLOAT = range(3)
+PATTERN_GPU, PATTERN_GPU_TIMESLICE = 0, 1
 
 _STATUS_NAMES = {
     GF_ERR_BAD_SHAPE: "BAD_SHAPE",
@@ -24,6 +27,7 @@ _STATUS_NAMES = {
     GF_ERR_HIP: "HIP_ERROR",
     GF_ERR_NOT_CONVERGED: "NOT_CONVERGED",
     GF_ERR_NO_DEVICE: "NO_DEVICE",
+    GF_ERR_IO: "IO_ERROR",
 }
 
 
@@ -60,7 +64,7 @@ _d = C.c_double
 _pl = C.POINTER(gf_lattice)
 _pi4 = C.POINTER(C.c_int)
 
-# name -> (restype, argtypes); every function declared in include/su3gf.h
+# name -> (restype, argtypes); every function declared in include/*.h
 SIGNATURES = {
     "gf_last_error": (C.c_char_p, []),
     "gf_version": (C.c_char_p, []),
@@ -98,6 +102,12 @@ SIGNATURES = {
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_make_config": (_i, [_pi4, _d, _u32, _u32, C.POINTER(_d)]),
+    # include/su3gf_io.h
+    "gf_linkfile_create": (_i, [_i, _i, _i, C.POINTER(_vp)]),
+    "gf_linkfile_destroy": (None, [_vp]),
+    "gf_linkfile_load": (_i, [_vp, C.c_char_p, _pl, _vp]),
+    "gf_linkfile_save": (_i, [_vp, C.c_char_p, _pl, _vp, _i]),
+    "gf_linkfile_read_header": (_i, [_vp, C.c_char_p, _pl]),
 }
 
 _lib = None

@@ -29,7 +29,7 @@ HIP_SOURCES = [
     ("su3gf_gauge.hip", []),
     ("su3gf_api.hip", []),
 ]
-HOST_SOURCES = ["su3gf_config.cpp"]
+HOST_SOURCES = ["su3gf_config.cpp", "su3gf_io.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "su3gf_kernels.h"
+#include "su3gf_error.h"
 
 using namespace su3gf;
 
@@ -135,6 +136,13 @@ hipError_t pinned_state(DevState **out)
 }
 
 }  // namespace
+
+// the error state of the C ABI, for the host-only translation units
+gf_status su3gf::set_error(gf_status st, const char *msg)
+{
+    g_err = msg;
+    return st;
+}
 
 struct gf_stats {
     int dims[4];

@@ -8,9 +8,9 @@
 // stops below `precision`, a final projectSU3 and best-copy selection by gff.
 //
 // Lattice extents are compile-time constants, as in the reference
-// (-D_X_=.. -D_T_=.. -DDOUBLEPRECISION).  Configurations come from the
-// library's synthetic generator (--sigma/--cfgseed/--haar) or setHot
-// (--hotgaugefield 1); file input/output is not part of this build.
+// (-D_X_=.. -D_T_=.. -DDOUBLEPRECISION).  Configurations are read from files
+// (--fbasename given), made by setHot (--hotgaugefield 1), or else come from
+// the library's synthetic generator (--sigma/--cfgseed/--haar).
 //
 // Options keep the reference's names and defaults
 // (apps/program_options/ProgramOptions.hxx:229-266):
@@ -18,15 +18,24 @@
 //   --reproject 100  --seed 1  --sasteps 1000 --samin 0.01 --samax 0.4
 //   --microupdates 3  --ormaxiter 1000 --orparameter 1.7  --srmaxiter 1000
 //   --srparameter 1.7  --precision 1e-7  --checkprecision 100  --doSA 1
-//   --hotgaugefield 0   (plus --sigma 0.3 --cfgseed 1 --haar 2 for the input)
-// The last line printed is "RESULT gff=<gff> theta=<theta> or_sweeps=<n> ..."
+//   --hotgaugefield 0
+//   files: --ftype PLAIN|HEADERONLY|VOGT|ILDG|QCDSTAG  --fbasename <path>
+//   --fending .vogt  --fnumberformat 1  --fstartnumber 0  --fstepnumber 1
+//   --nconf 1  --fappendix gaugefixed_  --reinterpret STANDARD|DOUBLE|FLOAT
+//   --output_conf <path>  --output_ending ""  (MAG: --output_SA_functional
+//   <path> --save_each 0).  File i is <fbasename><i><fending> with i padded
+//   to fnumberformat digits, its output <output_conf><fappendix><i><output_ending>
+//   (program_options/FileIterator.hxx:45-95).
+// The last line printed per copy is "RESULT gff=<gff> theta=<theta> or_sweeps=<n> ..."
 // (one per timeslice for Coulomb).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -38,6 +47,15 @@
 #include "culgt/PhiloxWrapper.hxx"
 #include "culgt/SiteIndex.hxx"
 #include "culgt/U1xU1KernelsSU3.hxx"
+#include "culgt/FileHeaderOnly.hxx"
+#include "culgt/FilePlain.hxx"
+#include "culgt/FileVogt.hxx"
+#include "culgt/GpuPattern.hxx"
+#include "culgt/LinkFile.hxx"
+#include "culgt/SiteCoord.hxx"
+#include "culgt/filetype_typedefs.h"
+#include "culgt/ildg.hxx"
+#include "culgt/qcdstag.hxx"
 
 namespace {
 
@@ -48,6 +66,13 @@ struct Options {
     float samin = 0.01f, samax = 0.4f, orparameter = 1.7f, srparameter = 1.7f, precision = 1e-7f;
     double sigma = 0.3;
     unsigned cfgseed = 1, haar = 2;
+    // files (ProgramOptions.hxx:229-250)
+    FileType ftype = PLAIN;
+    bool ftype_set = false;
+    ReinterpretReal reinterpret = STANDARD;
+    std::string fbasename, fending = ".vogt", fappendix = "gaugefixed_", output_conf, output_ending;
+    std::string output_SA_functional;
+    int fnumberformat = 1, fstartnumber = 0, fstepnumber = 1, nconf = 1, save_each = 0;
 };
 
 Options parse(int argc, char **argv)
@@ -73,8 +98,92 @@ Options parse(int argc, char **argv)
     if (kv.count("sigma")) o.sigma = atof(kv["sigma"].c_str());
     if (kv.count("cfgseed")) o.cfgseed = (unsigned)atol(kv["cfgseed"].c_str());
     if (kv.count("haar")) o.haar = (unsigned)atol(kv["haar"].c_str());
+    auto S = [&](const char *k, std::string &v) { if (kv.count(k)) v = kv[k]; };
+    S("fbasename", o.fbasename); S("fending", o.fending); S("fappendix", o.fappendix);
+    S("output_conf", o.output_conf); S("output_ending", o.output_ending);
+    S("output_SA_functional", o.output_SA_functional);
+    I("fnumberformat", o.fnumberformat); I("fstartnumber", o.fstartnumber); I("fstepnumber", o.fstepnumber);
+    I("nconf", o.nconf); I("save_each", o.save_each);
+    if (kv.count("ftype")) {
+        std::istringstream in(kv["ftype"]);
+        in >> o.ftype;
+        o.ftype_set = true;
+    }
+    if (kv.count("reinterpret")) {
+        std::istringstream in(kv["reinterpret"]);
+        in >> o.reinterpret;
+    }
     return o;
 }
+
+// program_options/FileIterator.hxx:45-95: file number `id` padded to
+// fnumberformat digits
+std::string padded(int id, int width)
+{
+    std::ostringstream s;
+    s.width(width);
+    s.fill('0');
+    s << id;
+    return s.str();
+}
+
+typedef StandardPattern<SiteCoord<4, NO_SPLIT>, 4, 3> Standard;
+
+// The apps' file handling (LandauGaugeFixingSU3_4D.cu:118-121,147-183,296-323):
+// one LinkFile per type, kept across load and save so that a HEADERONLY or
+// VOGT header goes back out with the gauge-fixed links; ILDG and QCDSTAG
+// through their functions.  Mem is GpuPattern (4D gauges) or
+// GpuPatternTimeslice (Coulomb).
+template <class Mem> struct ConfigFiles {
+    LinkFile<FileHeaderOnly, Standard, Mem, SiteCoord<4, FULL_SPLIT> > headeronly;
+    LinkFile<FileVogt, Standard, Mem, SiteCoord<4, FULL_SPLIT> > vogt;
+    LinkFile<FilePlain, Standard, Mem, SiteCoord<4, FULL_SPLIT> > plain;
+    explicit ConfigFiles(ReinterpretReal r) : headeronly(r), vogt(r), plain(r) {}
+    static const bool slices = Mem::memory_pattern == GF_PATTERN_GPU_TIMESLICE;
+
+    bool load(FileType t, const std::string &name, Real *U)
+    {
+        SiteCoord<4, FULL_SPLIT> s(HOST_CONSTANTS::SIZE);
+        switch (t) {
+        case VOGT: return vogt.load(s, name, U);
+        case PLAIN: return plain.load(s, name, U);
+        case HEADERONLY: return headeronly.load(s, name, U);
+        case ILDG:
+            if (slices) readILDG_timeslice(s, name.c_str(), HOST_CONSTANTS::SIZE, U);
+            else readILDG(s, name.c_str(), HOST_CONSTANTS::SIZE, U);
+            return true;
+        case QCDSTAG:
+            return slices ? readQCDSTAG_timeslice(s, name.c_str(), HOST_CONSTANTS::SIZE, U)
+                          : readQCDSTAG(s, name.c_str(), HOST_CONSTANTS::SIZE, U);
+        }
+        return false;
+    }
+
+    bool save(FileType t, const std::string &in, const std::string &out, Real *U, int steps)
+    {
+        SiteCoord<4, FULL_SPLIT> s(HOST_CONSTANTS::SIZE);
+        switch (t) {
+        case VOGT: return vogt.save(s, out, U);
+        case PLAIN: return plain.save(s, out, U);
+        case HEADERONLY: return headeronly.save(s, out, U);
+        case ILDG:
+            if (slices) writeILDG_timeslice(s, in.c_str(), out.c_str(), HOST_CONSTANTS::SIZE, U, steps);
+            else writeILDG(s, in.c_str(), out.c_str(), HOST_CONSTANTS::SIZE, U, steps);
+            return true;
+        case QCDSTAG:
+            return slices ? writeQCDSTAG_timeslice(s, out.c_str(), HOST_CONSTANTS::SIZE, U)
+                          : writeQCDSTAG(s, out.c_str(), HOST_CONSTANTS::SIZE, U);
+        }
+        return false;
+    }
+};
+
+// what a run does besides fixing: MAG's per-copy functional file and the
+// saving of each better copy (MAGaugeFixingSU3_4D.cu:151-154,336,347-374)
+struct RunHooks {
+    FILE *functional = nullptr;
+    std::function<void(int copy, Real *U)> save_better;
+};
 
 #define HIPCHECK(x)                                                                  \
     do {                                                                             \
@@ -99,13 +208,12 @@ std::vector<Real> make_config(const Options &o, const int dims[4], int64_t vol)
 // ---------------------------------------------------------------- 4D gauges
 // LandauGaugeFixingSU3_4D.cu:184-286 (Landau), U1xU1GaugeFixingSU3_4D.cu (U1xU1 SA
 // schedule), MAGaugeFixingSU3_4D.cu:199-345 (MAG).
-template <class GType> int run4d(const Options &o)
+// U: the host configuration (GpuPattern); on return it holds the best copy.
+template <class GType> int run4d(const Options &o, std::vector<Real> &U, const RunHooks &hooks)
 {
     const int threadsPerBlock = NSB * 8;
     const int numBlocks = (int)(V / 2 / NSB);
     const size_t bytes = (size_t)V * 72 * sizeof(Real);
-    const int dims[4] = {Nt, Nx, Ny, Nz};
-    std::vector<Real> U = o.hot ? std::vector<Real>((size_t)V * 72) : make_config(o, dims, V);
     Real *dU;
     HIPCHECK(hipMalloc(&dU, bytes));
     // the reference's neighbour table (LandauGaugeFixingSU3_4D.cu:107-116);
@@ -224,12 +332,14 @@ template <class GType> int run4d(const Options &o)
         if (mag) {
             gaugeStats.generateGaugeQuality();
             printf("%d,%.17g\n", copy, gaugeStats.getCurrentGff());
+            if (hooks.functional) fprintf(hooks.functional, "%d,%.17g\n", copy, gaugeStats.getCurrentGff());
         }
         const bool better = gaugeStats.getCurrentGff() > bestGff;
         if (better) {
             printf("FOUND BETTER COPY\n");
             bestGff = gaugeStats.getCurrentGff();
             HIPCHECK(hipMemcpy(U.data(), dU, bytes, hipMemcpyDeviceToHost));
+            if (hooks.save_better && copy < o.gaugecopies - 1) hooks.save_better(copy, U.data());
         } else {
             printf("NO BETTER COPY\n");
         }
@@ -247,15 +357,10 @@ template <class GType> int run4d(const Options &o)
 // CoulombGaugeFixingSU3_4D.cu:137-317: one timeslice after the other; slice t
 // and slice t-1 are copied to the device, fixed, and written back if better.
 // The host array is in the timeslice layout (GpuPatternTimeslice.hxx:57-61).
-int run_coulomb(const Options &o)
+// 4D GpuPattern array -> the timeslice layout (GpuPatternTimeslice.hxx:57-61)
+std::vector<Real> to_timeslices(const std::vector<Real> &U4)
 {
-    const int threadsPerBlock = NSB * 8;
-    const int numBlocks = (int)(V3 / 2 / NSB);
-    const int64_t tsSize = V3 * 72;  // reals per timeslice
-    const size_t bytes = (size_t)tsSize * sizeof(Real);
-    const int dims[4] = {Nt, Nx, Ny, Nz};
-    // 4D generator output -> timeslice layout
-    std::vector<Real> U4 = o.hot ? std::vector<Real>((size_t)V * 72) : make_config(o, dims, V);
+    const int64_t tsSize = V3 * 72;
     std::vector<Real> U((size_t)V * 72);
     for (int t = 0; t < Nt; t++)
         for (int x = 0; x < Nx; x++)
@@ -267,6 +372,17 @@ int run_coulomb(const Options &o)
                     const int64_t i3 = ((x + y + z) % 2 == 0) ? n3 / 2 : n3 / 2 + V3 / 2;
                     for (int k = 0; k < 72; k++) U[(size_t)(t * tsSize + i3 + V3 * k)] = U4[(size_t)(i4 + V * k)];
                 }
+    return U;
+}
+
+// U: the host configuration in the timeslice layout; on return each slice
+// holds its best copy.
+int run_coulomb(const Options &o, std::vector<Real> &U)
+{
+    const int threadsPerBlock = NSB * 8;
+    const int numBlocks = (int)(V3 / 2 / NSB);
+    const int64_t tsSize = V3 * 72;  // reals per timeslice
+    const size_t bytes = (size_t)tsSize * sizeof(Real);
     Real *dUtUp, *dUtDw;
     HIPCHECK(hipMalloc(&dUtUp, bytes));
     HIPCHECK(hipMalloc(&dUtDw, bytes));
@@ -340,20 +456,76 @@ int run_coulomb(const Options &o)
     return 0;
 }
 
+int run_one(const Options &o, std::vector<Real> &U, const RunHooks &hooks)
+{
+    if (o.gauge == "landau") return run4d<LandauKernelsSU3>(o, U, hooks);
+    if (o.gauge == "u1xu1") return run4d<U1xU1KernelsSU3>(o, U, hooks);
+    if (o.gauge == "mag") return run4d<MAGKernelsSU3>(o, U, hooks);
+    return run_coulomb(o, U);
+}
+
+// every file of the FileIterator: load, fix, save (the apps' main loops)
+template <class Mem> int run_files(const Options &o)
+{
+    ConfigFiles<Mem> files(o.reinterpret);
+    const bool mag = o.gauge == "mag";
+    for (int id = o.fstartnumber; id < o.fstartnumber + o.nconf * o.fstepnumber; id += o.fstepnumber) {
+        const std::string in = o.fbasename + padded(id, o.fnumberformat) + o.fending;
+        const std::string out = o.output_conf + o.fappendix + padded(id, o.fnumberformat) + o.output_ending;
+        std::vector<Real> U((size_t)V * 72);
+        printf("loading %s\n", in.c_str());
+        if (!files.load(o.ftype, in, U.data())) {
+            printf("Error while loading. Trying next file.\n");
+            break;  // the reference leaves the loop here
+        }
+        printf("File loaded.\n");
+        RunHooks hooks;
+        FILE *functional = nullptr;
+        if (mag && !o.output_SA_functional.empty()) {
+            const std::string f = o.output_SA_functional + "_" + padded(id, o.fnumberformat);
+            functional = fopen(f.c_str(), "w");
+            if (functional) fprintf(functional, "copy,functional\n");
+            hooks.functional = functional;
+        }
+        if (mag && o.save_each)
+            hooks.save_better = [&](int copy, Real *Ub) {
+                files.save(o.ftype, in, out + "_" + std::to_string(copy + 1), Ub, o.sasteps);
+            };
+        const int rc = run_one(o, U, hooks);
+        if (functional) fclose(functional);
+        if (rc != 0) return rc;
+        if (!(mag && o.save_each)) {
+            printf("saving %s\n", out.c_str());
+            if (!files.save(o.ftype, in, out, U.data(), o.sasteps)) return 1;
+        }
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     const Options o = parse(argc, argv);
+    if (o.gauge != "landau" && o.gauge != "u1xu1" && o.gauge != "mag" && o.gauge != "coulomb") {
+        fprintf(stderr, "unknown gauge %s\n", o.gauge.c_str());
+        return 2;
+    }
     try {
-        if (o.gauge == "landau") return run4d<LandauKernelsSU3>(o);
-        if (o.gauge == "u1xu1") return run4d<U1xU1KernelsSU3>(o);
-        if (o.gauge == "mag") return run4d<MAGKernelsSU3>(o);
-        if (o.gauge == "coulomb") return run_coulomb(o);
+        if (!o.hot && !o.fbasename.empty()) {
+            if (!o.ftype_set) {
+                fprintf(stderr, "Filetype not set to a known value. Exiting...\n");
+                return 1;
+            }
+            if (o.gauge == "coulomb") return run_files<GpuPatternTimeslice<SiteCoord<4, FULL_SPLIT>, 4, 3> >(o);
+            return run_files<GpuPattern<SiteCoord<4, FULL_SPLIT>, 4, 3> >(o);
+        }
+        const int dims[4] = {Nt, Nx, Ny, Nz};
+        std::vector<Real> U = o.hot ? std::vector<Real>((size_t)V * 72) : make_config(o, dims, V);
+        if (o.gauge == "coulomb" && !o.hot) U = to_timeslices(U);
+        return run_one(o, U, RunHooks());
     } catch (const std::exception &e) {
         fprintf(stderr, "error: %s\n", e.what());
         return 1;
     }
-    fprintf(stderr, "unknown gauge %s\n", o.gauge.c_str());
-    return 2;
 }

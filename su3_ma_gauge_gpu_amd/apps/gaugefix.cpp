@@ -26,6 +26,9 @@
 //   <path> --save_each 0).  File i is <fbasename><i><fending> with i padded
 //   to fnumberformat digits, its output <output_conf><fappendix><i><output_ending>
 //   (program_options/FileIterator.hxx:45-95).
+//   Also --devicenumber/-D, -m for --nconf, --help, and a config file of
+//   "key = value" lines (positional, or --config-file); the command line
+//   overrides it (ProgramOptions.hxx:221-286).
 // The last line printed per copy is "RESULT gff=<gff> theta=<theta> or_sweeps=<n> ..."
 // (one per timeslice for Coulomb).
 #include <hip/hip_runtime.h>
@@ -73,19 +76,75 @@ struct Options {
     std::string fbasename, fending = ".vogt", fappendix = "gaugefixed_", output_conf, output_ending;
     std::string output_SA_functional;
     int fnumberformat = 1, fstartnumber = 0, fstepnumber = 1, nconf = 1, save_each = 0;
+    int devicenumber = -1;  // -1: the runtime's default device
 };
+
+std::string trim(const std::string &t)
+{
+    const size_t a = t.find_first_not_of(" \t\r"), b = t.find_last_not_of(" \t\r");
+    return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+}
+
+// "key = value" lines, '#' starting a comment (the config-file format of
+// boost::program_options::parse_config_file used by ProgramOptions.hxx:258-262)
+void read_config_file(const std::string &path, std::map<std::string, std::string> &kv)
+{
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) {
+        fprintf(stderr, "cannot open config file %s\n", path.c_str());
+        exit(2);
+    }
+    char line[4096];
+    while (fgets(line, sizeof line, f)) {
+        std::string l(line);
+        const size_t hash = l.find('#');
+        if (hash != std::string::npos) l.resize(hash);
+        const size_t eq = l.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string k = trim(l.substr(0, eq));
+        if (!k.empty() && !kv.count(k)) kv[k] = trim(l.substr(eq + 1));  // the command line wins
+    }
+    fclose(f);
+}
+
+const char *const kHelp =
+    "Usage: gaugefix [options] [config-file]\n"
+    "  --gauge landau|u1xu1|mag|coulomb, and the reference apps' options:\n"
+    "  --config-file --devicenumber,-D --ftype --fbasename --fending --fnumberformat\n"
+    "  --fstartnumber --fstepnumber --nconf,-m --fappendix --reinterpret --output_conf\n"
+    "  --output_ending --output_SA_functional --save_each --doSA --hotgaugefield --seed\n"
+    "  --gaugecopies --randomtrafo --reproject --sasteps --samin --samax --microupdates\n"
+    "  --ormaxiter --orparameter --srmaxiter --srparameter --precision --checkprecision\n"
+    "  (--sigma --cfgseed --haar: the generated configuration when no file is read)\n";
 
 Options parse(int argc, char **argv)
 {
     Options o;
     std::map<std::string, std::string> kv;
-    for (int i = 1; i + 1 < argc; i += 2) {
-        if (strncmp(argv[i], "--", 2) != 0) {
-            fprintf(stderr, "unexpected argument %s\n", argv[i]);
+    std::string config;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "--help") {
+            printf("%s", kHelp);
+            exit(1);
+        }
+        std::string key;
+        if (a.rfind("--", 0) == 0) key = a.substr(2);
+        else if (a == "-D") key = "devicenumber";
+        else if (a == "-m") key = "nconf";
+        if (key.empty()) {  // the positional argument is the config file
+            config = a;
+            continue;
+        }
+        if (i + 1 >= argc) {
+            fprintf(stderr, "option %s needs a value\n", a.c_str());
             exit(2);
         }
-        kv[argv[i] + 2] = argv[i + 1];
+        kv[key] = argv[++i];
     }
+    if (kv.count("config-file")) config = kv["config-file"];
+    if (!config.empty()) read_config_file(config, kv);
+    o.devicenumber = kv.count("devicenumber") ? atoi(kv["devicenumber"].c_str()) : -1;
     auto I = [&](const char *k, int &v) { if (kv.count(k)) v = atoi(kv[k].c_str()); };
     auto F = [&](const char *k, float &v) { if (kv.count(k)) v = (float)atof(kv[k].c_str()); };
     if (kv.count("gauge")) o.gauge = kv["gauge"];
@@ -507,6 +566,7 @@ template <class Mem> int run_files(const Options &o)
 int main(int argc, char **argv)
 {
     const Options o = parse(argc, argv);
+    if (o.devicenumber >= 0) HIPCHECK(hipSetDevice(o.devicenumber));
     if (o.gauge != "landau" && o.gauge != "u1xu1" && o.gauge != "mag" && o.gauge != "coulomb") {
         fprintf(stderr, "unknown gauge %s\n", o.gauge.c_str());
         return 2;

@@ -70,3 +70,14 @@ def test_files_through_the_headers(exe, gf, tmp_path):
     for fmt in ("plain", "vogt", "headeronly", "qcdstag", "ildg", "missing", "options"):
         assert f"ok {fmt}" in out, (fmt, out)
     assert (tmp_path / "headeronly_out").read_bytes() == (tmp_path / "headeronly").read_bytes()
+
+
+def test_app_help_lists_the_reference_options():
+    exe = ROOT / "su3_ma_gauge_gpu_amd" / "apps" / "bin" / "gaugefix_x8t8_f64"
+    if not exe.exists():
+        pytest.skip("apps not built")
+    res = subprocess.run([str(exe), "--help"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 1
+    for opt in ("--ftype", "--fbasename", "--nconf", "--reinterpret", "--output_conf", "--save_each",
+                "--devicenumber", "--config-file", "--checkprecision"):
+        assert opt in res.stdout, opt

@@ -142,3 +142,22 @@ def test_mag_functional_file_and_save_each(gf, gpu, oracle, tmp_path):
         Uo = np.zeros(72 * V)
         gf.LinkFile(gf.FILE_VOGT).load(DIMS, tmp_path / f"gaugefixed_0_{k}", Uo)
         assert oracle.quality(MAG, DIMS, Uo.copy())[0] == pytest.approx(gffs[k - 1], rel=1e-12)
+
+
+def test_config_file_and_overrides(gf, gpu, tmp_path):
+    """A config file of "key = value" lines (comments after '#'), positional or
+    --config-file, with the command line taking precedence
+    (ProgramOptions.hxx:221-286): the same results as all options on the
+    command line."""
+    o = Options(gaugecopies=1, sasteps=10, ormaxiter=200, precision=1e-10, checkprecision=10)
+    want = parse_results(synthetic("landau", o))
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text("# gauge fixing\nsasteps = 10\normaxiter=200   # sweeps\nprecision = 1e-10\n"
+                   "checkprecision = 10\nsigma = 0.3\ncfgseed = 1\nhaar = 2\nseed = 99\n")
+    exe = BIN / "gaugefix_x8t8_f64"
+    for how in (["--seed", "1", str(cfg)], ["--config-file", str(cfg), "--seed", "1"]):
+        res = subprocess.run([str(exe), "--gauge", "landau", *how], capture_output=True, text=True, timeout=100)
+        assert res.returncode == 0, res.stderr[-2000:]
+        got = parse_results(res.stdout)
+        assert [(g["gff"], g["or_sweeps"], g["counter"]) for g in got] == [(w["gff"], w["or_sweeps"], w["counter"])
+                                                                           for w in want]

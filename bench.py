@@ -223,7 +223,7 @@ def workload_coulomb(args):
 
 def workload_mag(args):
     """MAG of a 48^4 lattice (sigma 0.35, seed 1, Haar seed 2): mixed (FP32
-    links with g, switch at theta < 1e-6) against all-FP64 to theta <
+    phase in --mode, switch at theta < 1e-6) against all-FP64 to theta <
     precision, then 1000 U1xU1 OR sweeps on the MAG result."""
     dims = (48, 48, 48, 48)
     V = int(np.prod(dims))
@@ -238,7 +238,7 @@ def workload_mag(args):
         _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
 
     reset()
-    m = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, prec, args.maxiter, mode=gf.MIXED_LINKS,
+    m = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, prec, args.maxiter, mode=args.mode,
                      refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
     print(f"bench: mag mixed {m}", file=sys.stderr, flush=True)
     u = gf.run_fast(lat, work, gf.U1XU1, args.omega, 1e-30, 1000, args.reproject, stream=stream)
@@ -250,7 +250,7 @@ def workload_mag(args):
                       m.sweeps * V / (m.ms_total / 1e3), "site-updates/s", m.ms_total, 1, 0,
                       {"workload": "mag-or-48x48x48x48-mixed+u1xu1", "dims": list(dims), "sigma": 0.35, "seed": 1,
                        "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
-                       "mode": MODE_NAMES[0]})
+                       "mode": MODE_NAMES[args.mode]})
     line.update({
         "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
         "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,

@@ -206,7 +206,7 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
  * (projected together with U), otherwise ignored.  *temperatures (optional)
  * receives the number of temperatures visited.
  * mode 0: the reference's sweeps on U in the precision of lat (asynchronous
- * on `stream`).  mode 1 (FP64 lat; Landau, U1xU1, or Coulomb on a 4D lattice,
+ * on `stream`).  mode 1 (FP64 lat; any gauge, Coulomb on a 4D lattice with
  * every timeslice at once): the gauge transformation is annealed in FP32 on a
  * read-only FP32 copy of U (same update rules and random numbers, FP32
  * arithmetic as in the reference's single-precision build), then applied to U
@@ -241,7 +241,8 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
  * every check32 sweeps): 1 FP32 links with gauge accumulation, 2 / 3 the
  * gauge-transformation sweep with FP32 / FP64 arithmetic (U is then the
  * read-only link copy), 4 the same as 2 on the links packed into 24-bit
- * fixed point.  U is modified by variants 0 and 1. */
+ * fixed point.  U is modified by variants 0 and 1.  Any gauge but Coulomb
+ * on a single timeslice. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
 
@@ -259,9 +260,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  *     as G U0 G^dagger (reprojected every `reproject` sweeps in between).
  *   mode GF_MIXED_GAUGE32 / GAUGE64: the FP32 copy of U0 is read only and
  *     the sweep updates g alone (links formed as g U0 g^dagger on the fly),
- *     in FP32 or FP64 arithmetic; Landau, Coulomb (4D lattice: every
- *     timeslice at once) and U1xU1.  refresh and reproject are not used in
- *     this phase.
+ *     in FP32 or FP64 arithmetic; every gauge (Coulomb on a 4D lattice:
+ *     every timeslice at once; MAG keeps each site's eight links in registers
+ *     and applies every SU(2) step to them).  refresh and reproject are not
+ *     used in this phase.
  *   mode GF_MIXED_GAUGE24: as GAUGE32 with U0 held in 24-bit fixed point
  *     (each real as round(v (2^23 - 1)), four to three 32-bit words): 36
  *     instead of 48 bytes per link, a quantisation error of at most 2^-24

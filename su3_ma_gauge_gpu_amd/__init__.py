@@ -433,11 +433,10 @@ def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, swit
     """FP32 sweeps accumulating a gauge transformation down to
     ``switch_precision`` (checked every ``check32`` sweeps), the transformation
     applied to the FP64 configuration, then FP64 fused-check sweeps to
-    ``precision``.  ``mode``: MIXED_GAUGE24 (default; MIXED_LINKS for MAG),
-    MIXED_GAUGE32, MIXED_GAUGE64 or MIXED_LINKS -- see gf_run_mixed in
-    include/su3gf.h."""
+    ``precision``.  ``mode``: MIXED_GAUGE24 (default), MIXED_GAUGE32,
+    MIXED_GAUGE64 or MIXED_LINKS -- see gf_run_mixed in include/su3gf.h."""
     if mode is None:
-        mode = MIXED_LINKS if gauge == MAG else MIXED_GAUGE24
+        mode = MIXED_GAUGE24
     r = _lib.gf_run_result()
     _lib.call("gf_run_mixed", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(switch_precision),
               C.c_float(precision), int(maxiter), int(mode), int(refresh), int(check32), int(reproject), C.byref(r),

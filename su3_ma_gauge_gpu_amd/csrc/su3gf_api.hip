@@ -558,7 +558,6 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     const bool gform = mode != 0;  // the gauge transformation is annealed (modes 1, 2)
     if (gform) {
         if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "modes 1 and 2 take an FP64 configuration");
-        if (gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
         if (gauge == GF_COULOMB && lat->dims[0] == 1)
             return fail(GF_ERR_BAD_ARG, "modes 1 and 2 anneal every timeslice of a 4D lattice at once");
     }
@@ -858,7 +857,6 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
     if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
-    if (variant >= 2 && gauge == GF_MAG) return fail(GF_ERR_BAD_ARG, "the gauge-transformation sweep has no MAG form");
     hipStream_t s = S(stream);
     FusedCtx fc;
     GF_TRY(fc.init(V / 2, s));
@@ -941,8 +939,6 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         return fail(GF_ERR_BAD_ARG, "gf_run_mixed fixes every timeslice of a 4D lattice at once");
     if (gauge == GF_COULOMB && mode == GF_MIXED_LINKS)
         return fail(GF_ERR_BAD_ARG, "Coulomb runs with the gauge-transformation modes");
-    if (gauge == GF_MAG && mode != GF_MIXED_LINKS)
-        return fail(GF_ERR_BAD_ARG, "MAG runs with GF_MIXED_LINKS (its update is not linear in the links)");
     memset(out, 0, sizeof *out);
     hipStream_t s = S(stream);
     gf_lattice l32 = *lat;

@@ -38,6 +38,8 @@
 // Delta = sum U_up - sum U_down = g P^dagger - Q g^dagger and the functional
 // Re tr of every link (each link has exactly one endpoint of either parity).
 // These are the quantities of LandauKernelsSU3.hxx:114-191.
+#include <type_traits>
+
 #include "su3gf_sweep.h"
 
 namespace su3gf {
@@ -575,6 +577,148 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     site_tail<A, G, ALGO, THETA>(a, fg, site_off, active, blk, lane, P, Q, Gx, rng);
 }
 
+// MAG on the gauge transformation.  MAG's subgroup quaternion is quadratic in
+// the links (GaugeFixingSubgroupStep.hxx:354-575), so there is no single matrix
+// M to update; instead wave mu keeps its two links U_up = g(x) U0_mu(x)
+// g(x+mu)^dagger and U_dn = g(x-mu) U0_mu(x-mu) g(x)^dagger in registers and
+// applies every SU(2) step to them (U_up <- G U_up, U_dn <- U_dn G^dagger), as
+// the link sweep does.  Per subgroup the waves exchange their (D, Im c, Re c)
+// through LDS; every wave adds the eight in the reference's ladder order (up
+// mu = 0..3, then down) and computes the same update from the same Philox
+// stream, so they agree bit for bit; wave 0 also updates and stores g(x).
+// theta (THETA = 1 before, 2 after the update) comes from the links as in the
+// link sweep's MAG check.
+template <typename A, int ALGO, int THETA, int LF>
+__global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
+{
+    constexpr int kX = 8 * 3 * 64;  // one exchange: [link 8][D, Im c, Re c][lane 64]
+    __shared__ A lds[2 * kX + (THETA ? kThetaVals * 4 * 64 : 0)];
+    A *const tlds = lds + 2 * kX;
+    if (a.state && a.state->done) return;  // converged: uniform early exit
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
+    const int64_t p = blk * 64 + lane;
+    const bool active = p < a.geo.Vh;
+    const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
+    int c[4], par;
+    site_coords(a.geo, site, c, par);
+    const uint32_t site_off = (uint32_t)site * 4u;
+    const Field fg = make_tiled_field(a.g, a.geo.V, 12);
+
+    M3<A> Gx;
+    load_rows01(fg, tile_off(site_off, 12), Gx);
+    third_row(Gx);
+    Philox rng;  // the same stream in every wave
+    if (ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) site_rng(a, p, rng);
+
+    M3<A> up, dn;
+    {
+        M3<A> X, Z;
+        direction_terms<A, GF_MAG, LF>(a, fg, mu, c, par, site_off, X, Z);
+        prod_ab(up, Gx, X);    // g(x) U0_mu(x) g(y)^dagger
+        prod_adag(dn, Z, Gx);  // g(z) U0_mu(z) g(x)^dagger
+    }
+
+    double th = 0, gf = 0;
+    auto theta_pass = [&]() {
+        A v[kThetaVals];
+        theta_contrib<A, GF_MAG>(up, dn, active, v);
+#pragma unroll
+        for (int k = 0; k < kThetaVals; k++) tlds[(k * 4 + mu) * 64 + lane] = v[k];
+        __syncthreads();
+        if (mu == 0) {
+            double s3[kThetaVals];
+#pragma unroll
+            for (int k = 0; k < kThetaVals; k++) {
+                const A *b = tlds + k * 4 * 64 + lane;
+                s3[k] = (double)(((b[0] + b[64]) + b[128]) + b[192]);
+            }
+            if (active) theta_site<GF_MAG>(s3, th, gf);
+        }
+    };
+    if (THETA == 1) theta_pass();
+
+    int buf = 0;
+    auto subgroup = [&](auto I_, auto J_) {
+        constexpr int I = decltype(I_)::value, J = decltype(J_)::value;
+        A vu[4], vd[4];
+        link_contrib<A, GF_MAG, I, J, true>(up, vu);
+        link_contrib<A, GF_MAG, I, J, false>(dn, vd);
+        A *x = lds + buf * kX;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            x[(mu * 3 + k) * 64 + lane] = vu[k];
+            x[((4 + mu) * 3 + k) * 64 + lane] = vd[k];
+        }
+        __syncthreads();
+        A cs[8][4];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) cs[m][k] = x[(m * 3 + k) * 64 + lane];
+            cs[m][3] = 0;
+        }
+        buf ^= 1;  // the next exchange uses the other buffer: a wave that races
+                   // ahead writes there, never into the one still being read
+        A q[4];
+        site_update<A, GF_MAG, ALGO, false>(cs, q, a.param, rng);
+        left_su2<A, I, J>(up, q);
+        const Hd<A> h = su2_dag(q);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const Cx<A> ci = col_i(h, dn.e[r][I], dn.e[r][J]), cj = col_j(h, dn.e[r][I], dn.e[r][J]);
+            dn.e[r][I] = ci;
+            dn.e[r][J] = cj;
+        }
+        if (mu == 0) left_su2<A, I, J>(Gx, q);
+    };
+    subgroup(std::integral_constant<int, 0>(), std::integral_constant<int, 2>());
+    subgroup(std::integral_constant<int, 1>(), std::integral_constant<int, 2>());
+    subgroup(std::integral_constant<int, 0>(), std::integral_constant<int, 1>());
+
+    if (THETA == 2) theta_pass();
+    if (mu != 0) return;
+    project_rows01(Gx);
+    if (active) store_rows01(fg, tile_off(site_off, 12), Gx);
+    if (THETA != 0) {
+        th = wave64_sum(th);
+        gf = wave64_sum(gf);
+        if (lane == 0) {
+            a.partial[2 * blk] = th;
+            a.partial[2 * blk + 1] = gf;
+        }
+    }
+}
+
+template <typename A, int ALGO, int THETA, int LF> void launch_k_mag(const GaugeSweepArgs &a, hipStream_t s)
+{
+    const dim3 grid((unsigned)gsweep_blocks(a.geo.Vh)), block(kGBlock);
+    hipLaunchKernelGGL((k_gsweep_mag<A, ALGO, THETA, LF>), grid, block, 0, s, a);
+}
+
+template <typename A, int LF> hipError_t launch_mag_lf(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
+{
+    if (algo == GF_OR) {
+        if (theta == 1) launch_k_mag<A, GF_OR, 1, LF>(a, s);
+        else if (theta == 2) launch_k_mag<A, GF_OR, 2, LF>(a, s);
+        else launch_k_mag<A, GF_OR, 0, LF>(a, s);
+    } else if (theta != 0) {
+        return hipErrorInvalidValue;
+    } else if (algo == GF_MICRO) {
+        launch_k_mag<A, GF_MICRO, 0, LF>(a, s);
+    } else if (algo == GF_SA) {
+        launch_k_mag<A, GF_SA, 0, LF>(a, s);
+    } else if (algo == GF_SR) {
+        launch_k_mag<A, GF_SR, 0, LF>(a, s);
+    } else if (algo == GF_RANDOM) {  // MAG's random step keeps q3 = 0 (GaugeFixingSubgroupStep.hxx:566-575)
+        launch_k_mag<A, GF_RANDOM, 0, LF>(a, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename A, int G, int ALGO, int THETA, int LF>
 void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 {
@@ -649,6 +793,7 @@ template <typename R, int SI> __global__ __launch_bounds__(256) void k_pack24(co
 
 template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
 {
+    if (gauge == GF_MAG) return a.links24 ? launch_mag_lf<A, 1>(algo, theta, a, s) : launch_mag_lf<A, 0>(algo, theta, a, s);
     if (gauge == GF_LANDAU) return launch_g<A, GF_LANDAU>(algo, theta, a, s);
     if (gauge == GF_COULOMB) return launch_g<A, GF_COULOMB>(algo, theta, a, s);
     if (gauge == GF_U1XU1) return launch_g<A, GF_U1XU1>(algo, theta, a, s);

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from app_schedule import to_timeslices
-from oracle_lib import COULOMB, LANDAU, OR, U1XU1, rows01
+from oracle_lib import COULOMB, LANDAU, MAG, OR, U1XU1, rows01
 
 pytestmark = pytest.mark.gpu
 
@@ -21,11 +21,12 @@ def _cfg(gf, dims, sigma=0.3, seed=1, haar=2):
 
 
 @pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6), (3, 2e-5)])
-@pytest.mark.parametrize("gauge", [LANDAU, U1XU1])
+@pytest.mark.parametrize("gauge", [LANDAU, U1XU1, MAG])
 def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
     # U1xU1 normalises (q0, q3) alone, which magnifies the FP32 storage
-    # rounding of small rotations: a looser bound
-    if gauge == U1XU1:
+    # rounding of small rotations: a looser bound; MAG's update is quadratic
+    # in the links and normalises (q0, q1, q2), likewise
+    if gauge in (U1XU1, MAG):
         tol *= 5
     """Three gauge-transformation sweeps (run_mixed with maxiter = 3 and a
     switch threshold never met), applied to U0 and projected, against three
@@ -129,6 +130,34 @@ def test_run_sa_schedule(gf, gpu, oracle, gauge, kw):
         assert res.gff == pytest.approx(want["gff"], rel=1e-9)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_run_sa_mag_gauge_modes(gf, gpu, oracle, mode):
+    """MAG's annealing schedule on the gauge transformation (five heatbath
+    steps per temperature): the same number of temperatures and RNG counters
+    as the link sweeps, the links change by a gauge transformation only
+    (plaquette to 1e-13), and the functional rises as with the link sweeps."""
+    from oracle_lib import to_matrices
+    from test_gpu_parity import _plaquette
+
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims, sigma=0.35)
+    lat = gf.Lattice(dims)
+    g0, _ = oracle.quality(MAG, dims, U.copy())
+    out = {}
+    for m in (0, mode):
+        d = gf.DeviceBuffer.from_numpy(U)
+        out[m] = gf.run_sa(lat, d, gf.MAG, 0.9, 0.6, 20, 2, 100, 1, 0, mode=m)
+        gf.CommonKernelsSU3(lat).projectSU3(lat.V // 32, 32, d)
+        Ua = d.download()
+        if m != 0:
+            assert _plaquette(to_matrices(Ua, V), dims) == pytest.approx(_plaquette(to_matrices(U, V), dims), rel=1e-13)
+        out[m] = out[m] + (oracle.quality(MAG, dims, Ua.copy())[0],)
+    assert out[mode][:2] == out[0][:2]
+    assert out[mode][2] > g0 + 0.02 and out[0][2] > g0 + 0.02
+    assert out[mode][2] == pytest.approx(out[0][2], abs=0.02)
+
+
 def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
     """Annealing the gauge transformation in FP32 (gf_run_sa mode 1, and mode 2
     on the links in 24-bit fixed point) changes the links only by a gauge
@@ -162,12 +191,14 @@ def test_run_sa_gauge_mode_is_a_gauge_transformation(gf, gpu, oracle):
     assert res[2] == pytest.approx(res[0], abs=0.02)
 
 
-def test_run_mixed_mag_links_mode(gf, gpu, oracle):
-    """MAG in the link mode of the mixed driver (FP32 links with g until
-    theta < 1e-6 or stagnation, then FP64 to 1e-10) ends on the FP64
-    oracle's maximum: functional to 1e-10, and the result is a gauge
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_run_mixed_mag(gf, gpu, oracle, mode):
+    """MAG through the mixed driver, in the link mode (FP32 links with g) and
+    on the gauge transformation (FP32 / 24-bit links, FP32 / FP64 arithmetic)
+    until theta < 1e-6 or stagnation, then FP64 to 1e-10: it ends on the FP64
+    oracle's maximum, functional to 1e-10, and the result is a gauge
     transformation of the input (plaquette to 1e-13)."""
-    from oracle_lib import MAG, to_matrices
+    from oracle_lib import to_matrices
     from test_gpu_parity import _plaquette
 
     dims = (4, 6, 4, 8)
@@ -175,7 +206,7 @@ def test_run_mixed_mag_links_mode(gf, gpu, oracle):
     U = _cfg(gf, dims)
     d = gf.DeviceBuffer.from_numpy(U)
     res = gf.run_mixed(gf.Lattice(dims), d, gf.MAG, 1.7, switch_precision=1e-6, precision=1e-10, maxiter=20000,
-                       mode=gf.MIXED_LINKS, check32=10)
+                       mode=mode, check32=10)
     Uo = U.copy()
     n, g, t = oracle.run_or(MAG, dims, Uo, 1.7, 1e-10, 20000, 1, 100)
     assert res.converged == 1 and res.theta < 1e-10

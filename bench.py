@@ -90,14 +90,15 @@ class Events:
         return float(ms.value)
 
 
-def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, variant: int) -> float:
-    """Average duration of one fused-check parity sweep kernel (the kernel the
-    drivers run), from HIP events around nsweeps back-to-back sweeps
-    (2 * nsweeps launches) on `stream`.  variant: 0 link sweep in the
-    precision of `lat`, 1 FP32 links + g, 2 / 3 gauge-transformation sweep in
-    FP32 / FP64 arithmetic."""
+def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, variant: int,
+                   gauge: int = gf.LANDAU) -> float:
+    """Average duration of one parity sweep kernel as the drivers run it, from
+    HIP events around nsweeps back-to-back sweeps (2 * nsweeps launches) on
+    `stream`.  variant: 0 link sweep in the precision of `lat` with the fused
+    check, 1 FP32 links + g, 2 / 3 gauge-transformation sweep in FP32 / FP64
+    arithmetic, 4 the FP32 gauge-transformation sweep on 24-bit links."""
     ms = C.c_float()
-    _lib.call("gf_time_sweeps", C.byref(lat.struct), C.c_void_p(U.ptr), gf.LANDAU, int(nsweeps), int(variant),
+    _lib.call("gf_time_sweeps", C.byref(lat.struct), C.c_void_p(U.ptr), int(gauge), int(nsweeps), int(variant),
               C.byref(ms), stream)
     return float(ms.value)
 
@@ -259,6 +260,24 @@ def workload_mag(args):
         "speedup_mixed_vs_fp64": f.ms_total / m.ms_total,
         "u1xu1_1000_sweeps_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
     })
+    # MAG's sweep kernels, one parity launch each (V/2 site-updates), and their
+    # algorithmic bytes as a fraction of 8 TB/s (bytes per site-update as in
+    # the Landau kernel table: the update rule does not change what is read)
+    lat32 = gf.Lattice(dims, gf.F32, gf.R12)
+    U32 = gf.DeviceBuffer(lat32.nbytes)
+    kms, frac = {}, {}
+    for var, name, bsu in ((4, "gauge_sweep_q24", 432), (2, "gauge_sweep_f32", 528), (1, "sweep_f32_r12_gauge", 864),
+                           (0, "sweep_f64_pattern18", 1536)):
+        if var:
+            _lib.call("gf_convert", C.byref(lat.struct), C.c_void_p(src.ptr), C.byref(lat32.struct),
+                      C.c_void_p(U32.ptr), 0, stream)
+            ms = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var, gf.MAG)
+        else:
+            reset()
+            ms = kernel_time_ms(lat, work, stream, args.kernel_sweeps, 0, gf.MAG)
+        kms[name] = ms
+        frac[name] = (V / 2) * bsu / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+    line.update({"kernel_ms": kms, "kernel_hbm_frac": frac})
     print(json.dumps(line), flush=True)
 
 

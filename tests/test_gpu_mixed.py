@@ -23,14 +23,14 @@ def _cfg(gf, dims, sigma=0.3, seed=1, haar=2):
 @pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6), (3, 2e-5)])
 @pytest.mark.parametrize("gauge", [LANDAU, U1XU1, MAG])
 def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
+    """Three gauge-transformation sweeps (run_mixed with maxiter = 3 and a
+    switch threshold never met), applied to U0 and projected, against three
+    FP64 OR sweeps of the oracle followed by projectSU3."""
     # U1xU1 normalises (q0, q3) alone, which magnifies the FP32 storage
     # rounding of small rotations: a looser bound; MAG's update is quadratic
     # in the links and normalises (q0, q1, q2), likewise
     if gauge in (U1XU1, MAG):
         tol *= 5
-    """Three gauge-transformation sweeps (run_mixed with maxiter = 3 and a
-    switch threshold never met), applied to U0 and projected, against three
-    FP64 OR sweeps of the oracle followed by projectSU3."""
     dims = (4, 6, 4, 8)
     V = int(np.prod(dims))
     U = _cfg(gf, dims)

@@ -211,7 +211,8 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
  * read-only FP32 copy of U (same update rules and random numbers, FP32
  * arithmetic as in the reference's single-precision build), then applied to U
  * in FP64 and U reprojected; returns when done.  mode 2: as mode 1 with the
- * copy of U in 24-bit fixed point (see GF_MIXED_GAUGE24). */
+ * copy of U in 24-bit fixed point (see GF_MIXED_GAUGE24).  Modes 1 and 2
+ * take at most 44,739,200 sites, as the gauge modes of gf_run_mixed. */
 gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
                     int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
                     void *stream);
@@ -268,6 +269,8 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  *     (each real as round(v (2^23 - 1)), four to three 32-bit words): 36
  *     instead of 48 bytes per link, a quantisation error of at most 2^-24
  *     per real, the size of an FP32 rounding error of an entry near 1.
+ * The gauge-transformation modes take at most 44,739,200 sites (80^4 fits;
+ * GF_ERR_BAD_SHAPE beyond), GF_MIXED_LINKS any volume below 2^31.
  * No reference counterpart (the reference's DPUPDATES is a compile-time
  * FP32/FP64 mix, gaugefixing/apps/Makefile:46-48). */
 gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, float switch_precision,

@@ -73,6 +73,17 @@ gf_status check_dims(const int dims[4], int64_t *V)
     return GF_OK;
 }
 
+// the gauge-transformation sweep's 32-bit buffer offsets bound its volume; the
+// link sweeps (GF_MIXED_LINKS, run_sa mode 0, gf_run_fast) have no such bound
+gf_status check_gauge_sweep_volume(int64_t V)
+{
+    if (gauge_sweep_fits(V)) return GF_OK;
+    return fail(GF_ERR_BAD_SHAPE,
+                "the gauge-transformation sweep takes at most 44739200 sites (%lld given); the link sweeps "
+                "(GF_MIXED_LINKS, annealing mode 0, gf_run_fast) have no such limit",
+                (long long)V);
+}
+
 gf_status check_lattice(const gf_lattice *lat, int64_t *V)
 {
     if (!lat) return fail(GF_ERR_BAD_ARG, "lattice descriptor is null");
@@ -555,6 +566,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     if (steps < 0 || micro < 0) return fail(GF_ERR_BAD_ARG, "negative step count");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
     if (mode < 0 || mode > 2) return fail(GF_ERR_BAD_ARG, "unknown annealing mode %d", mode);
+    if (mode > 0) GF_TRY(check_gauge_sweep_volume(V));
     const bool gform = mode != 0;  // the gauge transformation is annealed (modes 1, 2)
     if (gform) {
         if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "modes 1 and 2 take an FP64 configuration");
@@ -857,6 +869,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
     if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
+    if (variant >= 2) GF_TRY(check_gauge_sweep_volume(V));
     hipStream_t s = S(stream);
     FusedCtx fc;
     GF_TRY(fc.init(V / 2, s));
@@ -935,6 +948,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "the mixed driver takes an FP64 configuration");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
     if (mode < GF_MIXED_LINKS || mode > GF_MIXED_GAUGE24) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
+    if (mode != GF_MIXED_LINKS) GF_TRY(check_gauge_sweep_volume(V));
     if (gauge == GF_COULOMB && lat->dims[0] == 1)
         return fail(GF_ERR_BAD_ARG, "gf_run_mixed fixes every timeslice of a 4D lattice at once");
     if (gauge == GF_COULOMB && mode == GF_MIXED_LINKS)

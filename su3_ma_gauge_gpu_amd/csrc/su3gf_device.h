@@ -134,6 +134,11 @@ template <typename R> __device__ __forceinline__ LinkRef<R> link_ref(R *base, in
 // contiguous 3 KB tile instead of twelve planes V floats apart.  The buffer
 // holds gauge_floats(V) floats (whole tiles).
 __host__ __device__ inline int64_t gauge_floats(int64_t V) { return 12 * ((V + 63) / 64) * 64; }
+// The gauge-transformation sweep addresses g, and each direction of the links,
+// as one buffer with 32-bit byte offsets: 12 FP32 words per site in whole
+// tiles must stay below 2^31 bytes, i.e. at most 44,739,200 sites (80^4 fits,
+// 84^4 does not).
+__host__ __device__ inline bool gauge_sweep_fits(int64_t V) { return 4 * gauge_floats(V) < (int64_t(1) << 31); }
 __device__ __forceinline__ LinkRef<float> gauge_ref(float *g, int64_t i)
 {
     LinkRef<float> l;

@@ -849,8 +849,7 @@ hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, 
 
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
 {
-    // 32-bit byte offsets: a buffer of 12 FP32 components per site (whole tiles) must stay below 2^31 bytes
-    if (4 * gauge_floats(a.geo.V) >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    if (!gauge_sweep_fits(a.geo.V)) return hipErrorInvalidValue;
     GaugeSweepArgs b = a;
     const int64_t half_slice = a.geo.Vh / a.geo.N[0];  // sites of one parity in a timeslice
     if (b.remap && a.geo.N[0] % 8 == 0 && half_slice % 64 == 0) {

@@ -65,6 +65,29 @@ def test_sweep_rejects_bad_arguments(gf):
     assert lib.gf_sweep(C.byref(bad_layout), good, None, 0, 0, 0, C.c_float(1.7), 0, 0, 32, None) == 2
 
 
+def test_gauge_sweep_volume_limit(gf):
+    """The gauge-transformation sweep's fields use 32-bit byte offsets: its
+    drivers refuse more than 44,739,200 sites on the host, with the link
+    sweeps named as the way out, before touching the device."""
+    lib = gf._lib.load()
+    big = _lat(gf, (84, 84, 84, 84))  # 49,787,136 sites
+    good = C.c_void_p(0x1000)
+    out = gf._lib.gf_run_result()
+    for mode in (1, 2, 3):  # GF_MIXED_GAUGE32, GAUGE64, GAUGE24
+        rc = lib.gf_run_mixed(C.byref(big), good, 0, 1.7, 1e-6, 1e-12, 10, mode, 100, 10, 100, C.byref(out), None)
+        assert rc == 1, mode
+        assert "44739200" in gf._lib.last_error() and "49787136" in gf._lib.last_error()
+    ctr, temps = C.c_uint32(0), C.c_int(0)
+    for mode in (1, 2):
+        rc = lib.gf_run_sa(C.byref(big), good, None, 0, 1.4, 0.01, 10, 0, 100, mode, 1, C.byref(ctr), C.byref(temps),
+                           None)
+        assert rc == 1, mode
+    big32 = _lat(gf, (84, 84, 84, 84), 0, 12)
+    ms = C.c_float()
+    for variant in (2, 3, 4):
+        assert lib.gf_time_sweeps(C.byref(big32), good, 0, 10, variant, C.byref(ms), None) == 1, variant
+
+
 def test_stats_rejects_bad_arguments(gf):
     lib = gf._lib.load()
     h = C.c_void_p()

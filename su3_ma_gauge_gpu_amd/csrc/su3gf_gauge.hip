@@ -343,22 +343,29 @@ template <typename A> __device__ __forceinline__ void project_rows01(M3<A> &u)
     for (int k = 0; k < 3; k++) u.e[1][k] = u.e[1][k] * n1;
 }
 
-// theta_x and Re tr of the up and down links from sum U_up = G P^dagger and
-// sum U_down = Q G^dagger (G complete)
-template <int G, typename A>
-__device__ __forceinline__ void site_theta(const M3<A> &Gx, const M3<A> &P, const M3<A> &Q, double &th, double &gf)
+// theta_x and the Re tr of the site's eight links from M = S g(x)^dagger,
+// S = sum_mu g(x+mu) U0_mu(x)^dagger + g(x-mu) U0_mu(x-mu).  With A and B the
+// sums of the site's up links g(x) U0_mu(x) g(x+mu)^dagger and down links
+// g(x-mu) U0_mu(x-mu) g(x)^dagger, M = A^dagger + B, so
+// M - M^dagger = -(Delta - Delta^dagger) for Delta = A - B: theta_x, which
+// depends on Delta only through its traceless anti-Hermitian part
+// (LandauKernelsSU3.hxx:114-191), follows from M (the sign drops out of the
+// squares), and Re tr M = Re tr A + Re tr B (Landau, Coulomb, U1xU1).
+template <typename A> __device__ __forceinline__ void theta_from_m(const M3<A> &M, double &th, double &gf)
 {
-    M3<A> up, dn;
-    zero(up);
-    zero(dn);
-    mac_adag(up, Gx, P);
-    mac_adag(dn, Q, Gx);
-    A v[kThetaVals];
-    theta_contrib<A, G>(up, dn, true, v);
+    const A imtr3 = (M.e[0][0].y + M.e[1][1].y + M.e[2][2].y) / (A)3;
     double s[kThetaVals];
-#pragma unroll
-    for (int k = 0; k < kThetaVals; k++) s[k] = (double)v[k];
-    theta_site<G>(s, th, gf);
+    s[0] = (double)((A)2 * (M.e[0][0].y - imtr3));
+    s[1] = (double)((A)2 * (M.e[1][1].y - imtr3));
+    s[2] = (double)((A)2 * (M.e[2][2].y - imtr3));
+    s[3] = (double)(M.e[0][1].x - M.e[1][0].x);
+    s[4] = (double)(M.e[0][1].y + M.e[1][0].y);
+    s[5] = (double)(M.e[0][2].x - M.e[2][0].x);
+    s[6] = (double)(M.e[0][2].y + M.e[2][0].y);
+    s[7] = (double)(M.e[1][2].x - M.e[2][1].x);
+    s[8] = (double)(M.e[1][2].y + M.e[2][1].y);
+    s[9] = (double)((M.e[0][0].x + M.e[1][1].x) + M.e[2][2].x);
+    theta_site<GF_LANDAU>(s, th, gf);
 }
 
 // Block order.  The eight XCDs take blocks round-robin (block b runs on XCD
@@ -424,28 +431,28 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
     prod_ab(Z, T, W);  // g(z) U0_mu(z)
 }
 
-// Given P = sum_mu g(y) U0_mu(x)^dagger and Q = sum_mu g(z) U0_mu(z) (THETA),
-// or their sum in P, and g(x) rows 0,1 in Gx: the three subgroup updates of
-// g(x), its store, and the fused-check partial sums of the 64-site block blk.
-template <typename A, int G, int ALGO, int THETA>
+// Given S = sum_mu g(y) U0_mu(x)^dagger + g(z) U0_mu(z) (from sum_s, which
+// reads it again for the theta of the updated state) and g(x) rows 0,1 in
+// Gx: the three subgroup updates of g(x), its store, and the fused-check
+// partial sums of the 64-site block blk.
+template <typename A, int G, int ALGO, int THETA, typename SumS>
 __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &fg, uint32_t site_off, bool active,
-                                          int64_t blk, int lane, const M3<A> &P, const M3<A> &Q, M3<A> Gx,
-                                          Philox &rng)
+                                          int64_t blk, int lane, const SumS &sum_s, M3<A> Gx, Philox &rng)
 {
+    static_assert(THETA == 0 || ALGO != GF_RANDOM, "the random transformation forms no S to check");
     third_row(Gx);
 
     double th = 0, gf = 0;
-    M3<A> G0;  // theta of the state before the update is formed after it (shorter live ranges)
-    if (THETA == 1) G0 = Gx;
-
-    // M = (P + Q) Gx^dagger; the three subgroup steps read only the entries
-    // of M they need and update only the entries a later step reads
+    // M = S Gx^dagger; the three subgroup steps read only the entries of M
+    // they need and update only the entries a later step reads
     M3<A> M;
     zero(M);
     if (ALGO != GF_RANDOM) {
-        mac_adag(M, P, Gx);
-        if (THETA) mac_adag(M, Q, Gx);
+        M3<A> S;
+        sum_s(S);
+        mac_adag(M, S, Gx);
     }
+    if (THETA == 1 && active) theta_from_m(M, th, gf);  // the state before the update
     {   // subgroup (0,2): reads M00 M02 M20 M22; M00 M10 M02 M12 M22 are read later
         A q[4];
         quat<A, 0, 2>(M, q);
@@ -476,10 +483,13 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
     project_rows01(Gx);  // keep g unitary at the precision it is stored in
     if (active) store_rows01(fg, tile_off(site_off, 12), Gx);
 
-    if (THETA == 1 && active) site_theta<G>(G0, P, Q, th, gf);
-    if (THETA == 2 && active) {
+    if (THETA == 2 && active) {  // the state after the update, with g(x) as stored
         third_row(Gx);
-        site_theta<G>(Gx, P, Q, th, gf);
+        M3<A> S;
+        sum_s(S);
+        zero(M);
+        mac_adag(M, S, Gx);
+        theta_from_m(M, th, gf);
     }
     if (THETA != 0) {
         th = wave64_sum(th);
@@ -507,8 +517,7 @@ __device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, Phi
 template <typename A, int G, int ALGO, int THETA, int LF>
 __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 {
-    constexpr int kMats = THETA ? 2 : 1;  // P and Q apart for theta, else their sum
-    __shared__ A lds[kMats * 18 * 4 * 64];
+    __shared__ A lds[18 * 4 * 64];
     if (a.state && a.state->done) return;  // converged: uniform early exit
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
     const int lane = threadIdx.x & 63;
@@ -538,43 +547,30 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int k = 2 * (3 * i + j);
-                if (THETA) {
-                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x;
-                    lds[((k + 1) * 4 + mu) * 64 + lane] = -X.e[j][i].y;
-                    lds[((18 + k + 0) * 4 + mu) * 64 + lane] = Z.e[i][j].x;
-                    lds[((18 + k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y;
-                } else {
-                    lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
-                    lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
-                }
+                lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
+                lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
             }
         __syncthreads();
         if (mu != 0) return;
     }
 
-    // P = sum_mu g(y) U0_mu(x)^dagger, Q = sum_mu g(z) U0_mu(z) (theta), or S = P + Q
-    M3<A> P, Q;
-    if (ALGO != GF_RANDOM) {
+    // S = sum_mu g(y) U0_mu(x)^dagger + g(z) U0_mu(z), summed in the order mu = 0..3
+    const auto sum_s = [&](M3<A> &S) {
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int k = 2 * (3 * i + j);
-                A v[2], w[2];
+                A v[2];
 #pragma unroll
                 for (int r = 0; r < 2; r++) {
                     const A *b = lds + (k + r) * 4 * 64 + lane;
                     v[r] = ((b[0] + b[64]) + b[128]) + b[192];
-                    if (THETA) {
-                        const A *bq = lds + (18 + k + r) * 4 * 64 + lane;
-                        w[r] = ((bq[0] + bq[64]) + bq[128]) + bq[192];
-                    }
                 }
-                P.e[i][j] = cx(v[0], v[1]);
-                if (THETA) Q.e[i][j] = cx(w[0], w[1]);
+                S.e[i][j] = cx(v[0], v[1]);
             }
-    }
-    site_tail<A, G, ALGO, THETA>(a, fg, site_off, active, blk, lane, P, Q, Gx, rng);
+    };
+    site_tail<A, G, ALGO, THETA>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng);
 }
 
 // MAG on the gauge transformation.  MAG's subgroup quaternion is quadratic in

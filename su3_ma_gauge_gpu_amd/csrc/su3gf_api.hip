@@ -103,6 +103,20 @@ Geom make_geom(const int dims[4], int64_t V)
     g.fx = make_fastdiv((uint32_t)dims[1]);
     g.fy = make_fastdiv((uint32_t)dims[2]);
     g.fz = make_fastdiv((uint32_t)dims[3]);
+    g.slice_half = 0;
+    g.slice_bps = 0;
+    return g;
+}
+
+// Coulomb on a 4D lattice is checked per timeslice: the sweep kernels then
+// give each timeslice's sites of a parity blocks of their own, so that a
+// block's partial sums belong to one slice whatever the slice volume
+Geom per_slice(Geom g, int gauge)
+{
+    if (gauge == GF_COULOMB && g.N[0] > 1) {
+        g.slice_half = g.Vh / g.N[0];
+        g.slice_bps = (g.slice_half + 63) / 64;
+    }
     return g;
 }
 
@@ -675,9 +689,9 @@ struct FusedCtx {
     double *even() { return partial; }
     double *odd() { return partial + 2 * nblocks; }
     int64_t nblocks = 0;
-    gf_status init(int64_t Vh, hipStream_t s)
+    gf_status init(int64_t blocks, hipStream_t s)  // partial sums for up to `blocks` blocks per parity
     {
-        nblocks = sweep_blocks(Vh);
+        nblocks = blocks;
         HIP_TRY(dev.alloc(256 + sizeof(double) * 4 * nblocks, s));  // state, even, odd
         st = dev.as<DevState>();
         partial = (double *)((char *)dev.p + 256);
@@ -715,12 +729,7 @@ double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
 // Coulomb on a 4D lattice: each timeslice is its own problem and must meet
 // the precision on its own.  The per-slice check needs the 64-site blocks to
 // tile the slices; otherwise the lattice average is tested.
-int check_segments(const int dims[4], int gauge)
-{
-    if (gauge != GF_COULOMB || dims[0] == 1) return 1;
-    const int64_t half_slice = (int64_t)dims[1] * dims[2] * dims[3] / 2;
-    return half_slice % 64 == 0 ? dims[0] : 1;
-}
+int check_segments(const int dims[4], int gauge) { return gauge == GF_COULOMB && dims[0] > 1 ? dims[0] : 1; }
 
 // coarse_every > check_every: while the last theta the host has read is above
 // kCoarseFactor * tol (or before the first check), check only every
@@ -791,7 +800,7 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
     a.Udw = U;
     a.gacc = gacc;
     a.state = fc.st;
-    a.g = make_geom(lat->dims, V);
+    a.g = per_slice(make_geom(lat->dims, V), gauge);
     a.param = omega;
     a.nsb = 32;
     auto launch = [&](int parity, int theta, double *partial, int it) {
@@ -801,7 +810,7 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         return fast::launch_sweep<R>(lat->layout, gauge, GF_OR, theta, gacc != nullptr, a, s);
     };
     return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit,
-                      sweep_blocks(V / 2), fc, s, launch, after_sweep, sweeps_done, r);
+                      sweep_grid(a.g), fc, s, launch, after_sweep, sweeps_done, r);
 }
 
 gf_status final_quality(const gf_lattice *lat, const void *U, int gauge, gf_run_result *r, hipStream_t s)
@@ -831,7 +840,7 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     memset(out, 0, sizeof *out);
     hipStream_t s = S(stream);
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2, s));
+    GF_TRY(fc.init(sweep_grid(per_slice(make_geom(lat->dims, V), gauge)), s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -872,7 +881,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     if (variant >= 2) GF_TRY(check_gauge_sweep_volume(V));
     hipStream_t s = S(stream);
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2, s));
+    GF_TRY(fc.init(sweep_grid(make_geom(lat->dims, V)), s));
     Scratch gbuf, pbuf;
     struct {
         float *p = nullptr;
@@ -958,7 +967,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     gf_lattice l32 = *lat;
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
-    const Geom geo = make_geom(lat->dims, V);
+    const Geom geo = per_slice(make_geom(lat->dims, V), gauge);
     Scratch sU32, sg;
     HIP_TRY(sU32.alloc(mode == GF_MIXED_GAUGE24 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
     HIP_TRY(sg.alloc(sizeof(float) * gauge_floats(V), s));
@@ -966,7 +975,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         void *U32, *g;
     } b{sU32.p, sg.p};
     FusedCtx fc;
-    GF_TRY(fc.init(V / 2, s));
+    GF_TRY(fc.init(sweep_grid(geo), s));
     EventPair ev;
     hipEvent_t mid = nullptr;
     HIP_TRY(hipEventCreate(&ev.a));
@@ -1018,7 +1027,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         };
         auto none = [](int) -> gf_status { return GF_OK; };
         GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kCoarse * check32, kStall32,
-                          gsweep_blocks(V / 2),
+                          sweep_grid(geo),
                           fc, s, launch, none, &done32, out));
     }
     out->sweeps_fp32 = done32;

@@ -196,7 +196,32 @@ struct Geom {
     int N[4];          // {Nt, Nx, Ny, Nz}
     int64_t V, Vh;
     FastDiv fx, fy, fz;
+    // 0, or (Coulomb checked per timeslice) the blocks of 64 sites that each
+    // timeslice's slice_half sites of one parity get: no block spans two slices
+    int64_t slice_half, slice_bps;
 };
+
+// Blocks of a parity sweep: 64 consecutive sites of the parity each, or
+// slice_bps per timeslice (the last of a slice padded with idle lanes).
+__host__ __device__ inline int64_t sweep_grid(const Geom &g)
+{
+    return g.slice_bps ? (int64_t)g.N[0] * g.slice_bps : (g.Vh + 63) / 64;
+}
+
+// Index p (within the parity) of the site of lane `lane` of block b, and
+// whether the lane has a site; an idle lane's p is not a site of the block.
+__device__ __forceinline__ int64_t block_site(const Geom &g, int64_t b, int lane, bool &active)
+{
+    if (g.slice_bps == 0) {
+        const int64_t p = b * 64 + lane;
+        active = p < g.Vh;
+        return p;
+    }
+    const int64_t t = b / g.slice_bps;
+    const int64_t k = (b - t * g.slice_bps) * 64 + lane;
+    active = k < g.slice_half;
+    return t * g.slice_half + k;
+}
 
 // parity-split index -> coordinates (SiteCoord.hxx:168-196)
 __device__ __forceinline__ void site_coords(const Geom &g, int64_t idx, int c[4], int &par)

@@ -19,7 +19,7 @@ static inline unsigned blocks_for(int64_t threads) { return (unsigned)((threads 
 template <typename R, int G, int A, int S>
 static hipError_t launch_t(const SweepArgs<R> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_sweep<R, G, A, S, true, 0, false>), dim3((unsigned)sweep_blocks(a.g.Vh)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<R, G, A, S, true, 0, false>), dim3((unsigned)sweep_grid(a.g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

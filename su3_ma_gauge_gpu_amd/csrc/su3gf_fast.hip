@@ -11,7 +11,7 @@ namespace fast {
 template <typename R, int G, int A, int S, int T, bool GA>
 static hipError_t launch_t(const SweepArgs<R> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_sweep<R, G, A, S, false, T, GA>), dim3((unsigned)sweep_blocks(a.g.Vh)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<R, G, A, S, false, T, GA>), dim3((unsigned)sweep_grid(a.g)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -522,8 +522,8 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
     const int lane = threadIdx.x & 63;
     const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
-    const int64_t p = blk * 64 + lane;
-    const bool active = p < a.geo.Vh;
+    bool active;
+    const int64_t p = block_site(a.geo, blk, lane, active);
     const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
@@ -594,8 +594,8 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
-    const int64_t p = blk * 64 + lane;
-    const bool active = p < a.geo.Vh;
+    bool active;
+    const int64_t p = block_site(a.geo, blk, lane, active);
     const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
 
 template <typename A, int ALGO, int THETA, int LF> void launch_k_mag(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)gsweep_blocks(a.geo.Vh)), block(kGBlock);
+    const dim3 grid((unsigned)sweep_grid(a.geo)), block(kGBlock);
     hipLaunchKernelGGL((k_gsweep_mag<A, ALGO, THETA, LF>), grid, block, 0, s, a);
 }
 
@@ -718,7 +718,7 @@ template <typename A, int LF> hipError_t launch_mag_lf(int algo, int theta, cons
 template <typename A, int G, int ALGO, int THETA, int LF>
 void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)gsweep_blocks(a.geo.Vh)), block(kGBlock);
+    const dim3 grid((unsigned)sweep_grid(a.geo)), block(kGBlock);
     hipLaunchKernelGGL((k_gsweep<A, G, ALGO, THETA, LF>), grid, block, 0, s, a);
 }
 
@@ -798,7 +798,6 @@ template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const 
 
 }  // namespace
 
-int64_t gsweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 
 namespace {
 // 24-bit links -> FP32 rows 0,1 (compact layout), through the sweep kernels' decode
@@ -852,7 +851,7 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
         b.remap = 2;
         b.slice_blocks = half_slice / 64;
         b.slices_per_xcd = a.geo.N[0] / 8;
-    } else if (b.remap && gsweep_blocks(a.geo.Vh) % 8 == 0) {
+    } else if (b.remap && sweep_grid(a.geo) % 8 == 0) {
         b.remap = 1;
     } else {
         b.remap = 0;

@@ -48,7 +48,6 @@ template <typename R> struct QualityArgs {
     int crit_max;
 };
 
-inline int64_t sweep_blocks(int64_t Vh) { return (Vh + 63) / 64; }
 
 // Single-precision sweep on the gauge transformation (su3gf_gauge.hip): U0 is
 // read only, g(x) is updated; the links are g(x) U0_mu(x) g(x+mu)^dagger.
@@ -67,7 +66,6 @@ struct GaugeSweepArgs {
     int64_t nsb;       // sites per block of the reference launch (RNG key)
     int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR, SA, microcanonical)
 };
-int64_t gsweep_blocks(int64_t Vh);
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
 // 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
 // compute in FP64 on the FP32 data

@@ -338,8 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
 
     const int mu = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t p = (int64_t)blockIdx.x * 64 + lane;
-    const bool active = p < a.g.Vh;
+    bool active;
+    const int64_t p = block_site(a.g, blockIdx.x, lane, active);
     const int64_t site = (active ? p : 0) + (a.parity ? a.g.Vh : 0);
     int c[4], par;
     site_coords(a.g, site, c, par);

@@ -68,6 +68,59 @@ def _oracle_coulomb_slices(oracle, dims, U4, precision=1e-12, maxiter=20000):
     return out
 
 
+def _first_sweep_below(oracle, gauge, dims, U, X, maxiter=1000):
+    """The first FP64 OR sweep of the oracle (omega 1.7) after which theta <
+    X; for Coulomb the largest timeslice theta, every slice swept on its own
+    (the time links do not enter the Coulomb functional) in the order of the
+    4D checkerboard: a site's parity there is that of t + x + y + z, so on an
+    odd timeslice the even half-sweep updates the slice's odd sites first
+    (the reference's per-slice loop sweeps every slice even-first; both
+    orders reach the same maximum, along different paths)."""
+    if gauge == COULOMB:
+        dims3 = (1,) + tuple(dims[1:])
+        S = to_timeslices(U, dims)
+        slices = [(S[t].reshape(-1).copy(), S[t - 1].reshape(-1).copy()) for t in range(dims[0])]
+    else:
+        Uo = U.copy()
+    for n in range(1, maxiter + 1):
+        if gauge == COULOMB:
+            th = 0.0
+            for t, (up, dw) in enumerate(slices):
+                for parity in (t % 2, 1 - t % 2):
+                    oracle.sweep_coulomb(OR, dims3, up, dw, parity, 1.7)
+                th = max(th, oracle.quality(COULOMB, dims3, up)[1])
+        else:
+            for parity in (0, 1):
+                oracle.sweep(gauge, OR, dims, Uo, parity, 1.7)
+            th = oracle.quality(gauge, dims, Uo.copy())[1]
+        if th < X:
+            return n
+    raise AssertionError(f"the oracle's theta stays above {X} for {maxiter} sweeps")
+
+
+@pytest.mark.parametrize("gauge,mode", [(LANDAU, 0), (LANDAU, 1), (LANDAU, 2), (LANDAU, 3), (COULOMB, 1), (COULOMB, 3)])
+def test_fp32_check_ends_phase_where_fp64_theta_crosses(gf, gpu, oracle, gauge, mode):
+    """The FP32 phase's fused check -- theta_x from the update's own matrix
+    M = S g(x)^dagger in the gauge modes, from the links in the link mode --
+    ends the phase at the first sweep whose theta is below switch_precision:
+    with a check every sweep, within one sweep of where the FP64 oracle's
+    theta (the reference's quality, computed after every sweep) first falls
+    below it.  theta falls by 11-14% per sweep here (crossings at sweeps 93
+    and 82), so a wrong term, factor or sign would move the crossing by
+    several sweeps.  For Coulomb the check takes the largest timeslice theta;
+    at 6 x 4 x 8 a timeslice's sites of one parity (96) are not a whole
+    number of 64-site blocks, so each slice gets blocks of its own."""
+    dims = (4, 6, 4, 8)
+    U = _cfg(gf, dims)
+    X = 1e-6
+    n_star = _first_sweep_below(oracle, gauge, dims, U, X)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_mixed(gf.Lattice(dims), d, gauge, 1.7, switch_precision=X, precision=1e-30, maxiter=n_star + 20,
+                       mode=mode, check32=1)
+    print("crossing", gauge, mode, res.sweeps_fp32, n_star, flush=True)
+    assert abs(res.sweeps_fp32 - n_star) <= 1, (res.sweeps_fp32, n_star)
+
+
 @pytest.mark.parametrize("driver", ["fast", "mixed", "mixed24"])
 def test_coulomb_all_slices_at_once(gf, gpu, oracle, driver):
     """Coulomb gauge on an 8^3 x 8 lattice, all timeslices in one run (each

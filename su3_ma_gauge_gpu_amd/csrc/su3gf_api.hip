@@ -47,16 +47,6 @@ gf_status hip_fail(hipError_t e, const char *what)
         if (s_ != GF_OK) return s_;                               \
     } while (0)
 
-FastDiv make_fastdiv(uint32_t d)
-{
-    FastDiv f;
-    f.d = d;
-    f.s = 0;
-    while ((1ull << f.s) < d) f.s++;
-    f.m = (uint32_t)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
-    return f;
-}
-
 gf_status check_dims(const int dims[4], int64_t *V)
 {
     if (!dims) return fail(GF_ERR_BAD_ARG, "dims is null");
@@ -105,6 +95,7 @@ Geom make_geom(const int dims[4], int64_t V)
     g.fz = make_fastdiv((uint32_t)dims[3]);
     g.slice_half = 0;
     g.slice_bps = 0;
+    g.slice_div = make_fastdiv(1);
     return g;
 }
 
@@ -116,6 +107,7 @@ Geom per_slice(Geom g, int gauge)
     if (gauge == GF_COULOMB && g.N[0] > 1) {
         g.slice_half = g.Vh / g.N[0];
         g.slice_bps = (g.slice_half + 63) / 64;
+        g.slice_div = make_fastdiv((uint32_t)g.slice_bps);
     }
     return g;
 }

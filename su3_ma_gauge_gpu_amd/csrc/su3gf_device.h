@@ -192,6 +192,17 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv &f)
     return (__umulhi(n, f.m) + n) >> f.s;
 }
 
+// the multiplier and shift of fdiv for divisor d (1 <= d < 2^31)
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    f.d = d;
+    f.s = 0;
+    while ((1ull << f.s) < d) f.s++;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
+    return f;
+}
+
 struct Geom {
     int N[4];          // {Nt, Nx, Ny, Nz}
     int64_t V, Vh;
@@ -199,6 +210,7 @@ struct Geom {
     // 0, or (Coulomb checked per timeslice) the blocks of 64 sites that each
     // timeslice's slice_half sites of one parity get: no block spans two slices
     int64_t slice_half, slice_bps;
+    FastDiv slice_div;  // division by slice_bps
 };
 
 // Blocks of a parity sweep: 64 consecutive sites of the parity each, or
@@ -217,7 +229,7 @@ __device__ __forceinline__ int64_t block_site(const Geom &g, int64_t b, int lane
         active = p < g.Vh;
         return p;
     }
-    const int64_t t = b / g.slice_bps;
+    const int64_t t = fdiv((uint32_t)b, g.slice_div);  // block indices stay below 2^31
     const int64_t k = (b - t * g.slice_bps) * 64 + lane;
     active = k < g.slice_half;
     return t * g.slice_half + k;

@@ -376,14 +376,15 @@ template <typename A> __device__ __forceinline__ void theta_from_m(const M3<A> &
 // XCD's eighth -- Nt/8 timeslices -- with t innermost, so that the t
 // neighbours' g is read from L2 too: in plain order they are a timeslice
 // (V3/2 sites, megabytes of traffic) apart.
-__device__ __forceinline__ int64_t logical_block(int remap, int64_t slice_blocks, int slices_per_xcd)
+__device__ __forceinline__ int64_t logical_block(const GaugeSweepArgs &a)
 {
-    const int64_t b = blockIdx.x, n = gridDim.x;
-    if (remap == 0) return b;
-    const int64_t xcd = b & 7, k = b >> 3;
-    if (remap == 1) return xcd * (n >> 3) + k;
-    const int64_t t = xcd * slices_per_xcd + k % slices_per_xcd;  // timeslice
-    return t * slice_blocks + k / slices_per_xcd;
+    const uint32_t b = blockIdx.x, n = gridDim.x;  // below 2^31
+    if (a.remap == 0) return b;
+    const uint32_t xcd = b & 7, k = b >> 3;
+    if (a.remap == 1) return (int64_t)xcd * (n >> 3) + k;
+    const uint32_t q = fdiv(k, a.spx_div);
+    const int64_t t = (int64_t)xcd * a.slices_per_xcd + (k - q * (uint32_t)a.slices_per_xcd);  // timeslice
+    return t * a.slice_blocks + q;
 }
 
 // The update rule on the subgroup quaternion, with the U1xU1 restriction to
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     if (a.state && a.state->done) return;  // converged: uniform early exit
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
     const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
+    const int64_t blk = logical_block(a);
     bool active;
     const int64_t p = block_site(a.geo, blk, lane, active);
     const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
@@ -593,7 +594,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     if (a.state && a.state->done) return;  // converged: uniform early exit
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a.remap, a.slice_blocks, a.slices_per_xcd);
+    const int64_t blk = logical_block(a);
     bool active;
     const int64_t p = block_site(a.geo, blk, lane, active);
     const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
@@ -851,6 +852,7 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
         b.remap = 2;
         b.slice_blocks = half_slice / 64;
         b.slices_per_xcd = a.geo.N[0] / 8;
+        b.spx_div = make_fastdiv((uint32_t)b.slices_per_xcd);
     } else if (b.remap && sweep_grid(a.geo) % 8 == 0) {
         b.remap = 1;
     } else {

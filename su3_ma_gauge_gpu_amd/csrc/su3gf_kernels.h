@@ -61,6 +61,7 @@ struct GaugeSweepArgs {
     float param;       // OR parameter or SA temperature
     int remap;         // block order: 0 plain, else chosen by the launcher (XCD-aware)
     int slices_per_xcd;  // set by the launcher for remap = 2
+    FastDiv spx_div;     // division by slices_per_xcd
     int64_t slice_blocks;
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)

@@ -82,7 +82,11 @@ template <bool E, typename R> __device__ __forceinline__ void upd_or(R a[4], flo
 {
     const R ai = a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
     const R a0 = a[0] * a[0];
-    const R b = ((R)omega * a0 + ai) / (a0 + ai);
+    R b;
+    if constexpr (!E && sizeof(R) == 4)  // FP32 without the reference's order: reciprocal to 1 ulp
+        b = ((R)omega * a0 + ai) * __builtin_amdgcn_rcpf(a0 + ai);
+    else
+        b = ((R)omega * a0 + ai) / (a0 + ai);
     const R c = inv_sqrt<E>(a0 + b * b * ai);
     a[0] *= c;
     a[1] *= b * c;

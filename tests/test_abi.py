@@ -155,10 +155,12 @@ def _fastdiv(d):
     return m & 0xFFFFFFFF, s
 
 
-@pytest.mark.parametrize("d", [1, 2, 3, 4, 6, 8, 10, 12, 16, 20, 24, 30, 32, 48, 64, 96, 100, 128, 1000, 4096])
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 6, 8, 10, 12, 14, 16, 20, 24, 30, 32, 48, 63, 64, 96, 100, 128, 172, 256,
+                               1000, 4096, 65537, 2**30 + 3, 2**31 - 1])
 def test_fastdiv_formula(d):
-    """q = (umulhi(n, m) + n) >> s, as the kernels decode site indices, for
-    n < 2^31 (the volume limit checked by the library)."""
+    """q = (umulhi(n, m) + n) >> s, as the kernels decode site indices and
+    map blocks to timeslices, for n < 2^31 (the volume limit checked by the
+    library; block indices stay below it too)."""
     m, s = _fastdiv(d)
     rng = np.random.default_rng(d)
     ns = np.concatenate([np.arange(0, 5000), rng.integers(0, 2**31, 20000), [2**31 - 1, 2**31 - 2]]).astype(np.uint64)

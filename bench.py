@@ -398,6 +398,16 @@ def main():
                            stream=stream)
         print(f"bench: fp64 {fp64}", file=sys.stderr, flush=True)
 
+    # the drop-in path end to end: the reference app's OR loop (gf_run_or:
+    # reference-order kernels, the quality kernel and its reduction with a
+    # host read after every sweep, projectSU3 every `reproject` sweeps)
+    dropin = None
+    if not args.no_fp64:
+        reset()
+        dropin = gf.run_or(lat64, work, gf.LANDAU, args.omega, args.precision, args.maxiter, 1, args.reproject,
+                           stream=stream)
+        print(f"bench: drop-in run_or {dropin}", file=sys.stderr, flush=True)
+
     # the same run with the FP32 -> FP64 switch at theta < 1e-6, and the other
     # single-precision modes
     ns = None
@@ -504,6 +514,9 @@ def main():
             "time_to_theta_s": ns.ms_total / 1e3, "sweeps": ns.sweeps, "sweeps_fp32": ns.sweeps_fp32,
             "gff": ns.gff, "speedup_vs_fp64": fp64.ms_total / ns.ms_total},
         "mixed_modes": modes,
+        "dropin_run_or": None if dropin is None else {
+            "time_to_theta_s": dropin.ms_total / 1e3, "sweeps": dropin.sweeps, "gff": dropin.gff,
+            "theta": dropin.theta, "site_updates_per_s": dropin.sweeps * V / (dropin.ms_total / 1e3)},
         "site_updates_per_s_fp64": None if fp64 is None else fp64.sweeps * V / (fp64.ms_total / 1e3),
         "sweep_ms_in_run": {"fp32": t32, "fp64": t64},
         "kernel_ms": kms,

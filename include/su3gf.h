@@ -224,8 +224,12 @@ gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float 
  * flag on convergence that makes every later sweep of the run exit at once.
  * The host reads the flag once per 32 sweeps.  A 4D lattice with
  * gauge = GF_COULOMB fixes every timeslice at once (spatial links enter the
- * functional; the time links are transformed from both ends).  Every sweep is checked (the reference's
- * checkprecision = 1); out->sweeps is the index + 1 of the first sweep whose
+ * functional; the time links are transformed from both ends) until the
+ * largest timeslice theta meets the precision.  The 4D checkerboard takes
+ * the parity of t+x+y+z, so an odd timeslice has its odd sites updated
+ * first, where the reference's per-slice loop sweeps every slice even-first:
+ * the same maximum, reached along a slightly different path.  Every sweep is
+ * checked (the reference's checkprecision = 1); out->sweeps is the index + 1 of the first sweep whose
  * end state met the precision and out->gff / out->theta are that state's; the
  * array then holds that state plus one even half-sweep.  projectSU3 after
  * sweep i when i % reproject == 0.  No reference counterpart (the reference

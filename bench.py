@@ -343,8 +343,8 @@ def workload_gribov(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build is single-GPU: "
-              "see DESIGN.md for how the sweep would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the library runs one lattice "
+              "per GPU and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     if args.workload != "landau":
         {"coulomb": workload_coulomb, "mag": workload_mag, "gribov": workload_gribov}[args.workload](args)

@@ -225,7 +225,8 @@ def workload_coulomb(args):
 def workload_mag(args):
     """MAG of a 48^4 lattice (sigma 0.35, seed 1, Haar seed 2): mixed (FP32
     phase in --mode, switch at theta < 1e-6) against all-FP64 to theta <
-    precision, then 1000 U1xU1 OR sweeps on the MAG result."""
+    precision, then 1000 U1xU1 OR sweeps on the MAG result, in FP64 and in
+    mixed precision."""
     dims = (48, 48, 48, 48)
     V = int(np.prod(dims))
     U0 = gf.make_config(dims, 0.35, 1, 2)
@@ -242,8 +243,18 @@ def workload_mag(args):
     m = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, prec, args.maxiter, mode=args.mode,
                      refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
     print(f"bench: mag mixed {m}", file=sys.stderr, flush=True)
+    # U1xU1 on the MAG result, 1000 sweeps in FP64 and in mixed precision
+    # (the FP32 phase takes all of them: U1xU1's theta stays far above any
+    # switch threshold) from the same start
+    mag_fixed = gf.DeviceBuffer(lat.nbytes)
+    _lib.call("gf_memcpy_async", C.c_void_p(mag_fixed.ptr), C.c_void_p(work.ptr), C.c_size_t(lat.nbytes), stream)
     u = gf.run_fast(lat, work, gf.U1XU1, args.omega, 1e-30, 1000, args.reproject, stream=stream)
     print(f"bench: u1xu1 {u}", file=sys.stderr, flush=True)
+    _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(mag_fixed.ptr), C.c_size_t(lat.nbytes), stream)
+    um = gf.run_mixed(lat, work, gf.U1XU1, args.omega, 1e-30, 1e-30, 1000, mode=args.mode, refresh=args.refresh,
+                      check32=args.check32, reproject=args.reproject, stream=stream)
+    print(f"bench: u1xu1 mixed {um}", file=sys.stderr, flush=True)
+    del mag_fixed
     reset()
     f = gf.run_fast(lat, work, gf.MAG, args.omega, prec, args.maxiter, args.reproject, stream=stream)
     print(f"bench: mag fp64 {f}", file=sys.stderr, flush=True)
@@ -259,6 +270,8 @@ def workload_mag(args):
         "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
         "speedup_mixed_vs_fp64": f.ms_total / m.ms_total,
         "u1xu1_1000_sweeps_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
+        "u1xu1_1000_sweeps_mixed_s": um.ms_total / 1e3, "u1xu1_gff_mixed": um.gff,
+        "u1xu1_gff_rel_diff_mixed_vs_fp64": abs(um.gff - u.gff) / abs(u.gff),
     })
     # MAG's sweep kernels, one parity launch each (V/2 site-updates), and their
     # algorithmic bytes as a fraction of 8 TB/s (bytes per site-update as in

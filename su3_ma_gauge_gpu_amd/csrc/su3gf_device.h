@@ -276,9 +276,11 @@ __device__ __forceinline__ int64_t forward(const Geom &g, const int c[4], int pa
 }
 
 // ---------------------------------------------------------------- Philox
+// The words of the current block are named, not an array: an array indexed
+// by the running word count is kept in scratch memory by the compiler.
 struct Philox {
     uint32_t k0, k1, c0, c1;
-    uint32_t r[4];
+    uint32_t w0, w1, w2, w3;
     int local;
 };
 
@@ -313,7 +315,7 @@ __device__ __forceinline__ void philox_block(Philox &p)
     p.c0++;
     uint32_t c[4] = {p.c0, p.c1, 0x12345678u, 0xabcdef09u};
     philox10(c, p.k0, p.k1);
-    p.r[0] = c[0]; p.r[1] = c[1]; p.r[2] = c[2]; p.r[3] = c[3];
+    p.w0 = c[0]; p.w1 = c[1]; p.w2 = c[2]; p.w3 = c[3];
 }
 
 // uniform in (0,1): FP64 hands out the 64-bit words [1] then [0] of a block,
@@ -323,11 +325,11 @@ __device__ __forceinline__ double philox_rand(Philox &p, double)
     if (p.local == 0) {
         philox_block(p);
         p.local = 1;
-        const uint64_t w = (uint64_t)p.r[2] | ((uint64_t)p.r[3] << 32);
+        const uint64_t w = (uint64_t)p.w2 | ((uint64_t)p.w3 << 32);
         return (double)(1 + (w >> 11)) * 0x1.fffffffffffffp-54;
     }
     p.local = 0;
-    const uint64_t w = (uint64_t)p.r[0] | ((uint64_t)p.r[1] << 32);
+    const uint64_t w = (uint64_t)p.w0 | ((uint64_t)p.w1 << 32);
     return (double)(1 + (w >> 11)) * 0x1.fffffffffffffp-54;
 }
 
@@ -336,11 +338,14 @@ __device__ __forceinline__ float philox_rand(Philox &p, float)
     if (p.local == 0) {
         philox_block(p);
         p.local = 3;
-        return (float)(1 + (p.r[3] >> 8)) * 0x1.fffffep-25f;
+        return (float)(1 + (p.w3 >> 8)) * 0x1.fffffep-25f;
     }
+    // words 2, 1, 0 in turn: shift the next one into w3
     p.local--;
-    const uint32_t w = p.local == 2 ? p.r[2] : (p.local == 1 ? p.r[1] : p.r[0]);
-    return (float)(1 + (w >> 8)) * 0x1.fffffep-25f;
+    p.w3 = p.w2;
+    p.w2 = p.w1;
+    p.w1 = p.w0;
+    return (float)(1 + (p.w3 >> 8)) * 0x1.fffffep-25f;
 }
 
 // ------------------------------------------------------ quad reductions

@@ -121,14 +121,15 @@ def test_fp32_check_ends_phase_where_fp64_theta_crosses(gf, gpu, oracle, gauge, 
     assert abs(res.sweeps_fp32 - n_star) <= 1, (res.sweeps_fp32, n_star)
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 8, 8, 16)])
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 8, 8, 16), (16, 4, 4, 8)])
 @pytest.mark.parametrize("driver", ["fast", "mixed", "mixed24"])
 def test_coulomb_all_slices_at_once(gf, gpu, oracle, driver, dims):
-    """Coulomb gauge on an 8^3 x 8 and an 8 x 8 x 16 x 4 lattice, all
-    timeslices in one run (each slice must meet theta < 1e-12), against the
+    """Coulomb gauge on small 4D lattices, all timeslices in one run (each slice must meet theta < 1e-12), against the
     oracle fixing slice after slice: every slice's functional to 1e-10, every
-    slice converged.  The second shape takes the gauge-transformation sweep's
-    plain XCD block order (Nt = 4) over blocks laid out per timeslice."""
+    slice converged.  The shapes take the gauge-transformation sweep's
+    timeslice-walking XCD block order with one and two timeslices per XCD
+    (8^4, 16 x 4 x 4 x 8) and its plain XCD order (Nt = 4) over blocks laid out
+    per timeslice."""
     V = int(np.prod(dims))
     U = _cfg(gf, dims)
     d = gf.DeviceBuffer.from_numpy(U)

@@ -259,15 +259,16 @@ def test_run_fast_fixed_sweeps(gf, gpu, oracle, gauge):
 
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("dims,sigma,switch", [((8, 8, 8, 8), 0.3, 1e-6), ((12, 12, 12, 12), 0.25, 1e-6),
-                                               ((8, 8, 8, 8), 0.3, 1e-12), ((4, 8, 8, 16), 0.3, 1e-6)])
+                                               ((8, 8, 8, 8), 0.3, 1e-12), ((4, 8, 8, 16), 0.3, 1e-6),
+                                               ((16, 4, 4, 8), 0.3, 1e-6)])
 def test_run_mixed_matches_fp64_oracle(gf, gpu, oracle, dims, sigma, switch, mode):
     """FP32 phase (links + g, or g alone in FP32 / FP64 arithmetic, or on
     24-bit fixed-point links), then FP64: the final functional is the FP64
     oracle's to 1e-10 (same gauge orbit, same maximum).  The shapes take each
-    block order of the gauge-transformation sweep: 8^4 the timeslice-walking
-    XCD order (Nt a multiple of 8, a timeslice's sites of a parity whole
-    blocks), 4 x 8 x 8 x 16 the plain XCD order (32 blocks, Nt = 4), 12^4 the
-    launch order (162 blocks)."""
+    block order of the gauge-transformation sweep: 8^4 and 16 x 4 x 4 x 8 the
+    timeslice-walking XCD order (Nt a multiple of 8, a timeslice's sites of a
+    parity whole blocks; one and two timeslices per XCD), 4 x 8 x 8 x 16 the
+    plain XCD order (32 blocks, Nt = 4), 12^4 the launch order (162 blocks)."""
     U = _config(gf, dims, sigma, 1, 2)
     d = gf.DeviceBuffer.from_numpy(U)
     res = gf.run_mixed(gf.Lattice(dims), d, gf.LANDAU, 1.7, switch_precision=switch, precision=1e-12,

@@ -22,6 +22,8 @@
 // algorithms/{Or,Micro,Sa,Sr,Random}Update.hxx, {Landau,Coulomb,MAG,U1xU1}KernelsSU3.hxx
 // apply(), LandauKernelsSU3.hxx:114-191 (theta), CommonKernelsSU3.hxx:71-90.
 #pragma once
+#include <type_traits>
+
 #include "su3gf_kernels.h"
 
 namespace su3gf {
@@ -213,9 +215,35 @@ __device__ __forceinline__ void link_contrib(const M3<R> &u, R v[4])
     }
 }
 
+// The update of a site from its summed subgroup quaternion s (the update
+// rule, then the gauge-specific restriction and normalisation).
+template <typename R, int G, int A, bool E>
+__device__ __forceinline__ void finish_update(R s0, R s1, R s2, R s3, R q[4], float param, Philox &rng)
+{
+    if (G == GF_LANDAU || G == GF_COULOMB) {
+        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = s3;
+        calc_update<R, A, E>(q, param, rng);
+    } else if (G == GF_U1XU1) {
+        q[0] = s0; q[1] = 0; q[2] = 0; q[3] = s3;
+        calc_update<R, A, E>(q, param, rng);
+        q[1] = 0;
+        q[2] = 0;
+        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
+    } else {  // MAG (GaugeFixingSubgroupStep.hxx:529-575)
+        s1 *= (R)2;
+        s2 *= (R)2;
+        s0 = s0 + sqrt_(s0 * s0 + s1 * s1 + s2 * s2);
+        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = 0;
+        calc_update<R, A, E>(q, param, rng);
+        q[3] = 0;
+        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
+    }
+}
+
 // Reference accumulation of the eight contributions of one site in ladder
-// order; q[] receives the SU(2) element to apply (after the update rule and
-// the gauge-specific normalisation).
+// order (up mu = 0..3, then down); q[] receives the SU(2) element to apply.
 template <typename R, int G, int A, bool E>
 __device__ __forceinline__ void site_update(const R (&c)[8][4], R q[4], float param, Philox &rng)
 {
@@ -231,31 +259,43 @@ __device__ __forceinline__ void site_update(const R (&c)[8][4], R q[4], float pa
             if (G == GF_COULOMB && m == 0) continue;
             s0 += c[4 + m][0]; s1 += c[4 + m][1]; s2 += c[4 + m][2]; s3 += c[4 + m][3];
         }
-        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = s3;
-        calc_update<R, A, E>(q, param, rng);
     } else if (G == GF_U1XU1) {
 #pragma unroll
         for (int m = 0; m < 4; m++) { s0 += c[m][0]; s3 -= c[m][3]; }
 #pragma unroll
         for (int m = 0; m < 4; m++) { s0 += c[4 + m][0]; s3 += c[4 + m][3]; }
-        q[0] = s0; q[1] = 0; q[2] = 0; q[3] = s3;
-        calc_update<R, A, E>(q, param, rng);
-        q[1] = 0;
-        q[2] = 0;
-        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
-    } else {  // MAG (GaugeFixingSubgroupStep.hxx:529-575)
+    } else {
 #pragma unroll
         for (int m = 0; m < 8; m++) { s0 += c[m][0]; s1 += c[m][1]; s2 += c[m][2]; }
-        s1 *= (R)2;
-        s2 *= (R)2;
-        s0 = s0 + sqrt_(s0 * s0 + s1 * s1 + s2 * s2);
-        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = 0;
-        calc_update<R, A, E>(q, param, rng);
-        q[3] = 0;
-        const R n = inv_sqrt<E>(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        q[0] *= n; q[1] *= n; q[2] *= n; q[3] *= n;
     }
+    finish_update<R, G, A, E>(s0, s1, s2, s3, q, param, rng);
+}
+
+// One direction's share of the summed quaternion: its up link's contribution
+// u and its down link's d combined with the signs of the ladder sum above
+// (Coulomb's time direction contributes nothing).
+template <typename R, int G>
+__device__ __forceinline__ void direction_share(const R u[4], const R d[4], bool time_link, R p[4])
+{
+    if ((G == GF_COULOMB && time_link)) {
+        p[0] = p[1] = p[2] = p[3] = 0;
+    } else if (G == GF_MAG) {
+        p[0] = u[0] + d[0]; p[1] = u[1] + d[1]; p[2] = u[2] + d[2]; p[3] = 0;
+    } else {  // Landau, Coulomb, U1xU1 (which reads p[0] and p[3] only)
+        p[0] = u[0] + d[0]; p[1] = d[1] - u[1]; p[2] = d[2] - u[2]; p[3] = d[3] - u[3];
+    }
+}
+
+// The same update from the four directions' shares, added in the order
+// mu = 0..3 (the reference's order of the eight terms is not kept).
+template <typename R, int G, int A, bool E>
+__device__ __forceinline__ void site_update_dirs(const R (&c)[4][4], R q[4], float param, Philox &rng)
+{
+    const R s0 = ((c[0][0] + c[1][0]) + c[2][0]) + c[3][0];
+    const R s1 = ((c[0][1] + c[1][1]) + c[2][1]) + c[3][1];
+    const R s2 = ((c[0][2] + c[1][2]) + c[2][2]) + c[3][2];
+    const R s3 = ((c[0][3] + c[1][3]) + c[2][3]) + c[3][3];
+    finish_update<R, G, A, E>(s0, s1, s2, s3, q, param, rng);
 }
 
 // ------------------------------------------------------- fused theta / gff
@@ -328,15 +368,22 @@ __device__ __forceinline__ double wave64_sum(double v)
 // THETA: 0 none, 1 theta of the even sites before the update (even kernel),
 // 2 theta of the odd sites after the update (odd kernel); each block writes its
 // (theta, gff) sums to a.partial[2*block .. 2*block+1].
-// GACC: also left-multiply the site's gauge transformation g(x) (compact
-// FP32 rows 0,1 in a.gacc[idx + V*comp]) by the same SU(2) elements.
+// GACC: also left-multiply the site's gauge transformation g(x) (FP32 rows
+// 0,1 in 64-site tiles at a.gacc) by the same SU(2) elements.
+// EXACT exchanges the eight links' contributions and adds them in the
+// reference's ladder order; otherwise each wave exchanges its direction's
+// combined share (half the LDS traffic and registers), and the fused check
+// has an LDS area of its own, so that the even kernel's theta shares the first
+// exchange's barrier.
 template <typename R, int G, int A, int S, bool EXACT, int THETA, bool GACC>
 __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
 {
-    constexpr int kQ = 8 * 4 * 64;  // one exchange buffer: [link 8][comp 4][lane 64]
+    static_assert(!EXACT || THETA == 0, "the reference-order kernels carry no fused check");
+    constexpr int kTerms = EXACT ? 8 : 4;   // exchanged per site: links, or directions
+    constexpr int kQ = kTerms * 4 * 64;     // one exchange buffer: [term][comp 4][lane 64]
     constexpr int kT = kThetaVals * 4 * 64;
-    constexpr int kLds = (2 * kQ > kT ? 2 * kQ : kT);
-    __shared__ R lds[kLds];
+    __shared__ R lds[2 * kQ + (THETA ? kT : 0)];
+    R *const tlds = lds + 2 * kQ;
 
     if (a.state && a.state->done) return;  // converged: uniform early exit
 
@@ -387,57 +434,76 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
         philox_init(rng, tid, a.seed, a.counter);
     }
 
+    // fused check: each wave writes its direction's terms to the check's own
+    // LDS area, and wave 0 adds them at the end of the kernel, after a barrier
+    // that follows the writes (the area is written once per launch, so no
+    // barrier guards its reuse; the even kernel's writes precede the first
+    // exchange's barrier)
     double th = 0, gf = 0;
-    auto theta_pass = [&]() {
+    auto theta_write = [&]() {
         R v[kThetaVals];
         theta_contrib<R, G>(up, dn, active && !time_link, v);
 #pragma unroll
-        for (int k = 0; k < kThetaVals; k++) lds[(k * 4 + mu) * 64 + lane] = v[k];
-        __syncthreads();
-        if (mu == 0) {
-            double s[kThetaVals];
+        for (int k = 0; k < kThetaVals; k++) tlds[(k * 4 + mu) * 64 + lane] = v[k];
+    };
+    auto theta_read = [&]() {
+        if (mu != 0) return;
+        double s[kThetaVals];
 #pragma unroll
-            for (int k = 0; k < kThetaVals; k++)
-                s[k] = ((double)lds[(k * 4 + 0) * 64 + lane] + (double)lds[(k * 4 + 1) * 64 + lane]) +
-                       ((double)lds[(k * 4 + 2) * 64 + lane] + (double)lds[(k * 4 + 3) * 64 + lane]);
-            double t, g;
-            theta_site<G>(s, t, g);
-            th = wave64_sum(active ? t : 0.0);
-            gf = wave64_sum(active ? g : 0.0);
-        }
-        __syncthreads();
+        for (int k = 0; k < kThetaVals; k++)
+            s[k] = ((double)tlds[(k * 4 + 0) * 64 + lane] + (double)tlds[(k * 4 + 1) * 64 + lane]) +
+                   ((double)tlds[(k * 4 + 2) * 64 + lane] + (double)tlds[(k * 4 + 3) * 64 + lane]);
+        double t, g;
+        theta_site<G>(s, t, g);
+        th = wave64_sum(active ? t : 0.0);
+        gf = wave64_sum(active ? g : 0.0);
     };
 
-    if (THETA == 1) theta_pass();
+    if (THETA == 1) theta_write();
 
-#define SU3GF_SUBGROUP(I, J, BUF)                                                            \
-    {                                                                                        \
-        R* const bq = lds + (BUF) * kQ;                                                      \
-        R vu[4], vd[4];                                                                      \
-        link_contrib<R, G, I, J, true>(up, vu);                                              \
-        link_contrib<R, G, I, J, false>(dn, vd);                                             \
-        _Pragma("unroll") for (int k = 0; k < 4; k++)                                        \
-        {                                                                                    \
-            bq[((0 + mu) * 4 + k) * 64 + lane] = vu[k];                                      \
-            bq[((4 + mu) * 4 + k) * 64 + lane] = vd[k];                                      \
-        }                                                                                    \
-        __syncthreads();                                                                     \
-        R cc[8][4];                                                                          \
-        _Pragma("unroll") for (int m = 0; m < 8; m++)                                        \
-            _Pragma("unroll") for (int k = 0; k < 4; k++) cc[m][k] = bq[(m * 4 + k) * 64 + lane]; \
-        R q[4];                                                                              \
-        site_update<R, G, A, EXACT>(cc, q, a.param, rng);                                    \
-        left_mult<R, I, J>(up, q);                                                           \
-        right_mult_dag<R, I, J>(dn, q);                                                      \
-        if (GACC && own_g) {                                                                 \
-            const float qf[4] = {(float)q[0], (float)q[1], (float)q[2], (float)q[3]};        \
-            left_mult<float, I, J>(gt, qf);                                                  \
-        }                                                                                    \
-    }
-    SU3GF_SUBGROUP(0, 2, 0)
-    SU3GF_SUBGROUP(1, 2, 1)
-    SU3GF_SUBGROUP(0, 1, 0)
-#undef SU3GF_SUBGROUP
+    auto subgroup = [&](auto I_, auto J_, int buf) {
+        constexpr int I = decltype(I_)::value, J = decltype(J_)::value;
+        R *const bq = lds + buf * kQ;
+        R vu[4], vd[4];
+        link_contrib<R, G, I, J, true>(up, vu);
+        link_contrib<R, G, I, J, false>(dn, vd);
+        R q[4];
+        if constexpr (EXACT) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                bq[((0 + mu) * 4 + k) * 64 + lane] = vu[k];
+                bq[((4 + mu) * 4 + k) * 64 + lane] = vd[k];
+            }
+            __syncthreads();
+            R cc[8][4];
+#pragma unroll
+            for (int m = 0; m < 8; m++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) cc[m][k] = bq[(m * 4 + k) * 64 + lane];
+            site_update<R, G, A, EXACT>(cc, q, a.param, rng);
+        } else {
+            R pd[4];
+            direction_share<R, G>(vu, vd, time_link, pd);
+#pragma unroll
+            for (int k = 0; k < 4; k++) bq[(mu * 4 + k) * 64 + lane] = pd[k];
+            __syncthreads();
+            R cc[4][4];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) cc[m][k] = bq[(m * 4 + k) * 64 + lane];
+            site_update_dirs<R, G, A, EXACT>(cc, q, a.param, rng);
+        }
+        left_mult<R, I, J>(up, q);
+        right_mult_dag<R, I, J>(dn, q);
+        if (GACC && own_g) {
+            const float qf[4] = {(float)q[0], (float)q[1], (float)q[2], (float)q[3]};
+            left_mult<float, I, J>(gt, qf);
+        }
+    };
+    subgroup(std::integral_constant<int, 0>(), std::integral_constant<int, 2>(), 0);
+    subgroup(std::integral_constant<int, 1>(), std::integral_constant<int, 2>(), 1);
+    subgroup(std::integral_constant<int, 0>(), std::integral_constant<int, 1>(), 0);
 
     if (active) {
         store12(lu, up);
@@ -445,9 +511,11 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     }
     if (GACC && own_g) store12(lg, gt);
 
+    if (THETA == 1) theta_read();
     if (THETA == 2) {
-        __syncthreads();  // the last subgroup's readers are done with the buffers
-        theta_pass();
+        theta_write();
+        __syncthreads();
+        theta_read();
     }
     if (THETA != 0 && threadIdx.x == 0) {
         // thread 0 (wave 0) holds the block's sums; a separate check kernel

@@ -22,6 +22,40 @@ SHAPES = [(4, 4, 4, 4), (4, 6, 4, 8), (6, 4, 8, 4)]
 KERNELS = {LANDAU: "LandauKernelsSU3", MAG: "MAGKernelsSU3", U1XU1: "U1xU1KernelsSU3"}
 
 
+def test_reference_golden_lattice(gf, gpu):
+    """The drop-in kernels against the reference's own code with no oracle in
+    between: tests/golden/ref_vectors.npz holds what the reference's
+    SiteIndex / SU3 / Quaternion classes, composed into an FP64 Landau sweep
+    and compiled from the reference sources (oracle/ref_harness.cpp), made of
+    one lattice.  projectSU3, three OR sweeps (omega 1.7) and then a
+    microcanonical sweep through the C ABI give the same bits; the functional
+    and theta of the input and of the swept lattice agree to rounding (the
+    GPU reduction adds in another order: 1e-12 relative, 1e-14 absolute)."""
+    from pathlib import Path
+    g = np.load(Path(__file__).resolve().parent / "golden" / "ref_vectors.npz")
+    dims = tuple(int(x) for x in g["lat_dims"])
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims)
+    a, b = V // 2 // 32, 256
+    d = gf.DeviceBuffer.from_numpy(g["lat_in"].copy())
+    st = gf.GaugeFixingStats(gf.LandauKernelsSU3, d, lat)
+    st.generateGaugeQuality()
+    assert (st.getCurrentGff(), st.getCurrentA()) == pytest.approx(tuple(g["lat_quality_in"]), rel=1e-12, abs=1e-14)
+    P = gf.DeviceBuffer.from_numpy(g["lat_in"].copy())
+    gf.CommonKernelsSU3(lat).projectSU3(V // 32, 32, P)
+    assert np.array_equal(P.download(np.float64, 72 * V), g["lat_project"])
+    K = gf.LandauKernelsSU3(lat)
+    for _ in range(3):
+        for parity in (0, 1):
+            K.orStep(a, b, d, None, parity, 1.7)
+    assert np.array_equal(d.download(np.float64, 72 * V), g["lat_or3"])
+    st.generateGaugeQuality()
+    assert (st.getCurrentGff(), st.getCurrentA()) == pytest.approx(tuple(g["lat_quality_or3"]), rel=1e-12, abs=1e-14)
+    for parity in (0, 1):
+        K.microStep(a, b, d, None, parity)
+    assert np.array_equal(d.download(np.float64, 72 * V), g["lat_or3_micro1"])
+
+
 def _config(gf, dims, sigma=0.3, seed=1, haar=2):
     return gf.make_config(dims, sigma, seed, haar)
 

@@ -1,6 +1,7 @@
-"""The benchmark's own lattice, 32^4 Landau (sigma 0.25, seed 1, Haar seed 2),
-through size-independent properties: the CPU oracle would need minutes for
-one run at this size, so the checks compare the GPU paths with each other.
+"""The benchmark's own lattices through size-independent properties: 32^4
+Landau (sigma 0.25, seed 1, Haar seed 2) and the 32^3 x 64 Coulomb lattice.
+The CPU oracle would need minutes for one run at these sizes, so the checks
+compare the GPU paths with each other (the oracle only measures the results).
 
 * The fused FP64 driver (check every sweep) stops within one sweep of the
   drop-in reference loop (gf_run_or: the reference-order kernels, which are
@@ -54,3 +55,41 @@ def test_mixed_run_on_fp64_maximum_and_gauge_orbit(gf, gpu, config):
     p_in = _plaquette(to_matrices(config, V), DIMS)
     p_out = _plaquette(to_matrices(d.download(np.float64, config.size), V), DIMS)
     assert p_out == pytest.approx(p_in, rel=1e-12)
+
+
+COULOMB_DIMS = (64, 32, 32, 32)
+
+
+def test_coulomb_32x32x32x64_every_slice(gf, gpu, oracle):
+    """The Coulomb workload's lattice (32^3 x 64, sigma 0.3, seed 1, Haar
+    seed 2), every timeslice at once, in FP64 and in mixed precision: every
+    slice of both results has theta < 1e-12 by the oracle's per-slice quality,
+    the two agree slice by slice on the functional to 1e-10, and the mixed
+    result is a gauge transformation of the input (plaquette to 1e-12)."""
+    from app_schedule import to_timeslices
+    from oracle_lib import COULOMB
+
+    V = int(np.prod(COULOMB_DIMS))
+    dims3 = (1,) + COULOMB_DIMS[1:]
+    print("generating the 32^3 x 64 configuration", flush=True)
+    U = gf.make_config(COULOMB_DIMS, 0.3, 1, 2)
+    lat = gf.Lattice(COULOMB_DIMS)
+    slices = {}
+    for name in ("fp64", "mixed"):
+        d = gf.DeviceBuffer.from_numpy(U)
+        if name == "fp64":
+            r = gf.run_fast(lat, d, gf.COULOMB, 1.7, 1e-12, 20000, 100)
+        else:
+            r = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-12, precision=1e-12, maxiter=20000)
+        print(name, r, flush=True)
+        assert r.converged
+        out = d.download(np.float64, U.size)
+        S = to_timeslices(out, COULOMB_DIMS)
+        slices[name] = [oracle.quality(COULOMB, dims3, S[t].reshape(-1).copy()) for t in range(COULOMB_DIMS[0])]
+        assert max(th for _, th in slices[name]) < 1e-12, name
+        if name == "mixed":
+            p_in = _plaquette(to_matrices(U, V), COULOMB_DIMS)
+            p_out = _plaquette(to_matrices(out, V), COULOMB_DIMS)
+            assert p_out == pytest.approx(p_in, rel=1e-12)
+    for (g64, _), (gm, _) in zip(slices["fp64"], slices["mixed"]):
+        assert gm == pytest.approx(g64, rel=1e-10)

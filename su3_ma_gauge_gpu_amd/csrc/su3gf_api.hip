@@ -766,7 +766,9 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
         done = fc.host->done != 0;
         if (coarse && fc.host->checks > 0 && !(fc.host->theta > kCoarseFactor * tol)) coarse = false;
     }
-    r->checks += fc.host->checks;
+    // the host copy of the state is this loop's only after a chunk has run
+    // (a phase given no sweeps must not count the previous phase's checks again)
+    if (it > 0) r->checks += fc.host->checks;
     if (done) {
         *sweeps_done = fc.host->converged_sweep + 1;
         r->theta = fc.host->theta;

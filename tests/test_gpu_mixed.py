@@ -47,6 +47,19 @@ def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
     assert err < tol, err
 
 
+def test_check_count_when_fp32_phase_takes_every_sweep(gf, gpu):
+    """A mixed run whose FP32 phase uses up maxiter leaves the FP64 phase no
+    sweeps; the reported checks are the FP32 phase's alone.  With check32 = 5
+    and theta far above the switch threshold the phase checks every 25 sweeps:
+    after sweeps 25 and 50 of 60."""
+    dims = (4, 6, 4, 8)
+    d = gf.DeviceBuffer.from_numpy(_cfg(gf, dims))
+    res = gf.run_mixed(gf.Lattice(dims), d, LANDAU, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=60,
+                       mode=gf.MIXED_GAUGE24, check32=5)
+    assert res.sweeps == res.sweeps_fp32 == 60
+    assert res.checks == 2
+
+
 def _oracle_coulomb_slices(oracle, dims, U4, precision=1e-12, maxiter=20000):
     """Each timeslice fixed by the oracle on its own, OR with omega 1.7,
     checked every sweep (the reference's per-slice loop,

@@ -11,12 +11,13 @@
 // load instruction reads 64 consecutive sites of one link component (512 B in
 // FP64).  Lane (mu, p) owns the up link U_mu(x) and the down link U_mu(x-mu).
 // Per SU(2) subgroup each lane writes the contributions of its two links to
-// LDS; after one barrier every lane of the site reads the eight contributions
-// and adds them in the reference's order -- up links mu = 0..3, then down
-// links mu = 0..3 (GaugeFixingSubgroupStep.hxx:126-208) -- so the four waves
-// hold bit-identical sums and compute the same update, which each lane
-// applies to its own two links.  Two LDS buffers alternate between subgroups,
-// so one barrier per subgroup suffices.
+// LDS; after one barrier every lane of the site reads them and adds them --
+// the exact kernels all eight in the reference's order, up links mu = 0..3,
+// then down links mu = 0..3 (GaugeFixingSubgroupStep.hxx:126-208), the fast
+// kernels each direction's combined share -- so the four waves hold
+// bit-identical sums and compute the same update, which each lane applies to
+// its own two links.  Two LDS buffers alternate between subgroups, so one
+// barrier per subgroup suffices.
 //
 // Reference (paths relative to its src/): GaugeFixingSubgroupStep.hxx:74-705,
 // algorithms/{Or,Micro,Sa,Sr,Random}Update.hxx, {Landau,Coulomb,MAG,U1xU1}KernelsSU3.hxx

@@ -61,7 +61,8 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "device_info", "RunResult", "LinkFile",
+    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "batch_run_mixed", "batch_run_sa", "device_info",
+    "RunResult", "LinkFile",
 ]
 
 
@@ -425,6 +426,57 @@ def run_sa(lattice: Lattice, U, gauge: int = LANDAU, tmax: float = 0.4, tmin: fl
               int(steps), int(micro), int(reproject), int(mode), int(seed), C.byref(ctr), C.byref(ntemps),
               _ptr(stream))
     return int(ctr.value), int(ntemps.value)
+
+
+def _batch_args(lattices, Us, streams, max_concurrent):
+    if len(lattices) != len(Us):
+        raise ValueError("one configuration per lattice")
+    n = len(lattices)
+    lats = (_lib.gf_lattice * n)(*[l.struct for l in lattices])
+    ptrs = (C.c_void_p * n)(*[_ptr(u).value for u in Us])
+    strs = None
+    if streams is not None:
+        if len(streams) < max(1, max_concurrent):
+            raise ValueError("one stream per concurrent problem")
+        strs = (C.c_void_p * len(streams))(*[_ptr(s).value for s in streams])
+    return n, lats, ptrs, strs
+
+
+def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: float = 0.01, steps: int = 1000,
+                 micro: int = 3, reproject: int = 100, seeds=None, counters=None, mode: int = 0,
+                 max_concurrent: int = 4, streams=None) -> list[tuple[int, int]]:
+    """run_sa on independent 4D lattices in one call (gf_batch_run_sa in
+    include/su3gf.h): small lattices concurrently, each on its own stream
+    (``streams``: the caller's, one per concurrent problem, or None).
+    ``seeds`` and ``counters`` give each problem's RNG seed and starting
+    launch counter.  Returns each problem's (next RNG counter, number of
+    temperatures)."""
+    n, lats, ptrs, strs = _batch_args(lattices, Us, streams, max_concurrent)
+    seeds = list(seeds) if seeds is not None else [1] * n
+    counters = list(counters) if counters is not None else [0] * n
+    cs = (C.c_int * n)(*seeds)
+    ctr = (C.c_uint32 * n)(*counters)
+    temps = (C.c_int * n)()
+    _lib.call("gf_batch_run_sa", n, lats, ptrs, gauge, C.c_float(tmax), C.c_float(tmin), int(steps), int(micro),
+              int(reproject), int(mode), cs, ctr, temps, int(max_concurrent), strs)
+    return [(int(ctr[k]), int(temps[k])) for k in range(n)]
+
+
+def batch_run_mixed(lattices, Us, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,
+                    precision: float = 1e-12, maxiter: int = 20000, mode: int | None = None, refresh: int = 50,
+                    check32: int = 10, reproject: int = 100, max_concurrent: int = 4,
+                    streams=None) -> list[RunResult]:
+    """run_mixed on independent 4D lattices in one call (gf_batch_run_mixed in
+    include/su3gf.h): small lattices concurrently, each on its own stream
+    (``streams``: the caller's, one per concurrent problem, or None)."""
+    if mode is None:
+        mode = MIXED_GAUGE24
+    n, lats, ptrs, strs = _batch_args(lattices, Us, streams, max_concurrent)
+    out = (_lib.gf_run_result * n)()
+    _lib.call("gf_batch_run_mixed", n, lats, ptrs, gauge, C.c_float(omega), C.c_float(switch_precision),
+              C.c_float(precision), int(maxiter), int(mode), int(refresh), int(check32), int(reproject),
+              int(max_concurrent), strs, out)
+    return [_result(out[k]) for k in range(n)]
 
 
 def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,

@@ -101,6 +101,10 @@ SIGNATURES = {
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_batch_run_mixed": (_i, [_i, _pl, C.POINTER(_vp), _i, _f, _f, _f, _i, _i, _i, _i, _i, _i, C.POINTER(_vp),
+                                C.POINTER(gf_run_result)]),
+    "gf_batch_run_sa": (_i, [_i, _pl, C.POINTER(_vp), _i, _f, _f, _i, _i, _i, _i, C.POINTER(_i),
+                             C.POINTER(C.c_uint32), C.POINTER(_i), _i, C.POINTER(_vp)]),
     "gf_make_config": (_i, [_pi4, _d, _u32, _u32, C.POINTER(_d)]),
     # include/su3gf_io.h
     "gf_linkfile_create": (_i, [_i, _i, _i, C.POINTER(_vp)]),

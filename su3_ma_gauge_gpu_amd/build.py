@@ -29,7 +29,9 @@ HIP_SOURCES = [
     ("su3gf_gauge.hip", []),
     ("su3gf_api.hip", []),
 ]
-HOST_SOURCES = ["su3gf_config.cpp", "su3gf_io.cpp"]
+HOST_SOURCES = ["su3gf_config.cpp", "su3gf_io.cpp", "su3gf_batch.cpp"]
+
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -42,7 +44,8 @@ HIPCC_FLAGS = [
     "-munsafe-fp-atomics",
     f"-I{INCLUDE}",
 ]
-GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", f"-I{INCLUDE}"]
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}",
+             f"-I{INCLUDE}"]
 
 # The application on the drop-in headers (include/culgt): host code only, so
 # g++ with the HIP runtime headers; the lattice extents are compile-time
@@ -53,7 +56,6 @@ APP_VARIANTS = [
     ("gaugefix_x8t8_f32", ["-D_X_=8", "-D_T_=8"]),
     ("gaugefix_x32t32_f64", ["-D_X_=32", "-D_T_=32", "-DDOUBLEPRECISION"]),
 ]
-ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 
 def _hipcc() -> str:

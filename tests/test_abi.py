@@ -88,6 +88,33 @@ def test_gauge_sweep_volume_limit(gf):
         assert lib.gf_time_sweeps(C.byref(big32), good, 0, 10, variant, C.byref(ms), None) == 1, variant
 
 
+def test_batch_calls_check_every_problem_on_the_host(gf):
+    """The batch drivers reject a bad batch before any device call, naming
+    the problem; an empty batch is a no-op."""
+    lib = gf._lib.load()
+    good = _lat(gf, (4, 4, 4, 4))
+    out = (gf._lib.gf_run_result * 2)()
+    ptrs = (C.c_void_p * 2)(0x1000, 0x2000)
+    lats = (gf._lib.gf_lattice * 2)(good, _lat(gf, (4, 4, 3, 4)))
+    rc = lib.gf_batch_run_mixed(2, lats, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, None, out)
+    assert rc == 1 and "batch problem 1" in gf._lib.last_error()
+    lats_ok = (gf._lib.gf_lattice * 2)(good, good)
+    nullp = (C.c_void_p * 2)(0x1000, None)
+    assert lib.gf_batch_run_mixed(2, lats_ok, nullp, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, None, out) == 2
+    assert "batch problem 1" in gf._lib.last_error()
+    assert lib.gf_batch_run_mixed(-1, lats_ok, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, None, out) == 2
+    assert lib.gf_batch_run_mixed(2, None, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, None, out) == 2
+    assert lib.gf_batch_run_mixed(0, None, None, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, None, None) == 0
+    seeds = (C.c_int * 2)(1, 2)
+    ctr = (C.c_uint32 * 2)(0, 0)
+    assert lib.gf_batch_run_sa(2, lats, ptrs, 0, 1.0, 0.1, 10, 3, 100, 1, seeds, ctr, None, 4, None) == 1
+    assert lib.gf_batch_run_sa(2, lats_ok, ptrs, 0, 1.0, 0.1, 10, 3, 100, 1, None, ctr, None, 4, None) == 2
+    assert lib.gf_batch_run_sa(0, None, None, 0, 1.0, 0.1, 10, 3, 100, 1, None, None, None, 4, None) == 0
+    # the caller's streams: one per concurrent problem, none of them null
+    short = (C.c_void_p * 4)(0x10, 0x20, None, 0x40)
+    assert lib.gf_batch_run_mixed(2, lats_ok, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, short, out) == 2
+
+
 def test_stats_rejects_bad_arguments(gf):
     lib = gf._lib.load()
     h = C.c_void_p()

@@ -47,6 +47,45 @@ def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("own_streams", [False, True])
+def test_batch_runs_match_single_runs(gf, gpu, own_streams):
+    """gf_batch_run_sa and then gf_batch_run_mixed on three copies of an 8^4
+    configuration (Haar seeds 10-12), a 12^4 lattice and a 16 x 32^3 lattice
+    (4096 blocks per half-sweep: it runs after the others, alone), on streams
+    of the call's own or of the caller's: every problem's links, RNG counter,
+    temperatures and run results are those of the single-lattice calls with
+    the same arguments, bit for bit."""
+    import ctypes as C
+
+    streams = None
+    if own_streams:
+        streams = []
+        for _ in range(4):
+            s = C.c_void_p()
+            gf._lib.call("gf_stream_create", C.byref(s))
+            streams.append(s)
+    shapes = [(8, 8, 8, 8)] * 3 + [(12, 12, 12, 12), (16, 32, 32, 32)]
+    cfgs = [gf.make_config(d, 0.3, 1, 10 + k) for k, d in enumerate(shapes)]
+    lats = [gf.Lattice(d) for d in shapes]
+    seeds, counters = [3, 4, 5, 6, 7], [0, 1000, 2000, 3000, 4000]
+    bufs = [gf.DeviceBuffer.from_numpy(c) for c in cfgs]
+    sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.0, 0.05, 20, 3, 100, seeds, counters, mode=2, max_concurrent=4,
+                         streams=streams)
+    res = gf.batch_run_mixed(lats, bufs, gf.LANDAU, 1.7, 1e-8, 1e-11, 5000, check32=5, max_concurrent=4,
+                             streams=streams)
+    for k, dims in enumerate(shapes):
+        d = gf.DeviceBuffer.from_numpy(cfgs[k])
+        s1 = gf.run_sa(lats[k], d, gf.LANDAU, 1.0, 0.05, 20, 3, 100, seeds[k], counters[k], mode=2)
+        r1 = gf.run_mixed(lats[k], d, gf.LANDAU, 1.7, 1e-8, 1e-11, 5000, check32=5)
+        assert sa[k] == s1, (k, sa[k], s1)
+        fields = ("sweeps", "sweeps_fp32", "checks", "gff", "theta", "converged")
+        assert [getattr(res[k], f) for f in fields] == [getattr(r1, f) for f in fields], k
+        assert res[k].converged == 1
+        assert np.array_equal(bufs[k].download(np.float64, cfgs[k].size), d.download(np.float64, cfgs[k].size)), k
+    for s in streams or []:
+        gf._lib.call("gf_stream_destroy", s)
+
+
 def test_check_count_when_fp32_phase_takes_every_sweep(gf, gpu):
     """A mixed run whose FP32 phase uses up maxiter leaves the FP64 phase no
     sweeps; the reported checks are the FP32 phase's alone.  With check32 = 5

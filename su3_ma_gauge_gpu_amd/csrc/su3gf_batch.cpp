@@ -99,10 +99,17 @@ gf_status run_batch(int n, const gf_lattice *lats, int max_concurrent, void *con
     std::atomic<size_t> next{0};
     const int nthreads = std::min(std::max(1, max_concurrent), (int)small.size());
     std::vector<std::thread> pool;
-    for (int t = 0; t < nthreads; t++)
-        pool.emplace_back([&, t] {
-            for (size_t i; (i = next++) < small.size();) one(small[i], t);
-        });
+    try {
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                for (size_t i; (i = next++) < small.size();) one(small[i], t);
+            });
+    } catch (...) {
+        // a thread that cannot start leaves its share to the others, or to
+        // this thread if none started (no exception may cross the C ABI)
+    }
+    if (pool.empty())
+        for (size_t i; (i = next++) < small.size();) one(small[i], 0);
     for (auto &t : pool) t.join();
     for (int k : large) one(k, 0);
     for (int k = 0; k < n; k++)

@@ -579,16 +579,17 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 // M to update; instead wave mu keeps its two links U_up = g(x) U0_mu(x)
 // g(x+mu)^dagger and U_dn = g(x-mu) U0_mu(x-mu) g(x)^dagger in registers and
 // applies every SU(2) step to them (U_up <- G U_up, U_dn <- U_dn G^dagger), as
-// the link sweep does.  Per subgroup the waves exchange their (D, Im c, Re c)
-// through LDS; every wave adds the eight in the reference's ladder order (up
-// mu = 0..3, then down) and computes the same update from the same Philox
-// stream, so they agree bit for bit; wave 0 also updates and stores g(x).
+// the link sweep does.  Per subgroup each wave exchanges its direction's
+// (D, Im c, Re c), up and down link added, through LDS; every wave adds the
+// four in the order mu = 0..3 and computes the same update from the same
+// Philox stream, so they agree bit for bit; wave 0 also updates and stores
+// g(x).
 // theta (THETA = 1 before, 2 after the update) comes from the links as in the
 // link sweep's MAG check.
 template <typename A, int ALGO, int THETA, int LF>
 __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
 {
-    constexpr int kX = 8 * 3 * 64;  // one exchange: [link 8][D, Im c, Re c][lane 64]
+    constexpr int kX = 4 * 3 * 64;  // one exchange: [direction 4][D, Im c, Re c][lane 64]
     __shared__ A lds[2 * kX + (THETA ? kThetaVals * 4 * 64 : 0)];
     A *const tlds = lds + 2 * kX;
     if (a.state && a.state->done) return;  // converged: uniform early exit
@@ -639,19 +640,19 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     int buf = 0;
     auto subgroup = [&](auto I_, auto J_) {
         constexpr int I = decltype(I_)::value, J = decltype(J_)::value;
-        A vu[4], vd[4];
+        // each wave exchanges its direction's share: its up and down links'
+        // terms added (the reference's order of the eight terms is not kept)
+        A vu[4], vd[4], pd[4];
         link_contrib<A, GF_MAG, I, J, true>(up, vu);
         link_contrib<A, GF_MAG, I, J, false>(dn, vd);
+        direction_share<A, GF_MAG>(vu, vd, false, pd);
         A *x = lds + buf * kX;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            x[(mu * 3 + k) * 64 + lane] = vu[k];
-            x[((4 + mu) * 3 + k) * 64 + lane] = vd[k];
-        }
+        for (int k = 0; k < 3; k++) x[(mu * 3 + k) * 64 + lane] = pd[k];
         __syncthreads();
-        A cs[8][4];
+        A cs[4][4];
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
+        for (int m = 0; m < 4; m++) {
 #pragma unroll
             for (int k = 0; k < 3; k++) cs[m][k] = x[(m * 3 + k) * 64 + lane];
             cs[m][3] = 0;
@@ -659,7 +660,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
         buf ^= 1;  // the next exchange uses the other buffer: a wave that races
                    // ahead writes there, never into the one still being read
         A q[4];
-        site_update<A, GF_MAG, ALGO, false>(cs, q, a.param, rng);
+        site_update_dirs<A, GF_MAG, ALGO, false>(cs, q, a.param, rng);
         left_su2<A, I, J>(up, q);
         const Hd<A> h = su2_dag(q);
 #pragma unroll

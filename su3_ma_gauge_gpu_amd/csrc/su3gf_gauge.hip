@@ -47,6 +47,10 @@ namespace {
 
 constexpr int kGBlock = 256;  // 64 sites x 4 directions
 
+// Waves per block of the gauge-transformation sweep: one per direction that
+// enters S, so three for Coulomb, whose S leaves out the time direction.
+template <int G> constexpr int gsweep_waves() { return G == GF_COULOMB ? 3 : 4; }
+
 // The complex products below are written as explicit fused multiply-adds:
 // the compiler keeps a * b + c unfused when c is a sum formed elsewhere,
 // which costs a third more instructions in these issue-bound loops.
@@ -408,16 +412,11 @@ __device__ __forceinline__ void gauge_update(A q[4], float param, Philox &rng)
 
 // The direction-mu terms of site x (coordinates c, parity par):
 // X = U0_mu(x) g(y)^dagger and Z = g(z) U0_mu(z), y = x+mu, z = x-mu, so that
-// S_mu = X^dagger + Z (Coulomb: zero for mu = 0, the time direction).
-template <typename A, int G, int LF>
+// S_mu = X^dagger + Z.
+template <typename A, int LF>
 __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const Field &fg, int mu, const int c[4],
                                                 int par, uint32_t site_off, M3<A> &X, M3<A> &Z)
 {
-    if (G == GF_COULOMB && mu == 0) {
-        zero(X);
-        zero(Z);
-        return;
-    }
     M3<A> T, W;
     const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
     const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
@@ -511,16 +510,18 @@ __device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, Phi
     philox_init(rng, tid, a.seed, a.counter);
 }
 
-// Work mapping 0: a block of 256 threads covers 64 sites as four waves, wave mu
-// forming the direction-mu terms of its 64 sites (four 12-real loads per
-// lane, all coalesced).  The terms go through LDS; wave 0 adds them in the
-// order mu = 0..3 and runs the tail.
+// Work mapping 0: a block covers 64 sites with one wave per direction of S
+// (mu = 0..3; Coulomb mu = 1..3), each forming the direction-mu terms of its
+// 64 sites (four 12-real loads per lane, all coalesced).  The terms go through
+// LDS; wave 0 adds them in the order of mu and runs the tail.
 template <typename A, int G, int ALGO, int THETA, int LF>
 __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 {
-    __shared__ A lds[18 * 4 * 64];
+    constexpr int kW = gsweep_waves<G>();
+    __shared__ A lds[18 * kW * 64];
     if (a.state && a.state->done) return;  // converged: uniform early exit
-    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // one direction per wave
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mu = wave + (4 - kW);  // one direction per wave
     const int lane = threadIdx.x & 63;
     const int64_t blk = logical_block(a);
     bool active;
@@ -532,30 +533,30 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     const Field fg = make_tiled_field(a.g, a.geo.V, 12);
 
     M3<A> Gx;
-    if (mu == 0) load_rows01(fg, tile_off(site_off, 12), Gx);
+    if (wave == 0) load_rows01(fg, tile_off(site_off, 12), Gx);
 
     Philox rng;
     if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, rng);
 
     if (ALGO == GF_RANDOM) {
         // the random transformation does not depend on the links
-        if (mu != 0) return;
+        if (wave != 0) return;
     } else {
         M3<A> X, Z;
-        direction_terms<A, G, LF>(a, fg, mu, c, par, site_off, X, Z);
+        direction_terms<A, LF>(a, fg, mu, c, par, site_off, X, Z);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int k = 2 * (3 * i + j);
-                lds[((k + 0) * 4 + mu) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
-                lds[((k + 1) * 4 + mu) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
+                lds[((k + 0) * kW + wave) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
+                lds[((k + 1) * kW + wave) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
             }
         __syncthreads();
-        if (mu != 0) return;
+        if (wave != 0) return;
     }
 
-    // S = sum_mu g(y) U0_mu(x)^dagger + g(z) U0_mu(z), summed in the order mu = 0..3
+    // S = sum_mu g(y) U0_mu(x)^dagger + g(z) U0_mu(z), summed in the order of mu
     const auto sum_s = [&](M3<A> &S) {
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -565,8 +566,10 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
                 A v[2];
 #pragma unroll
                 for (int r = 0; r < 2; r++) {
-                    const A *b = lds + (k + r) * 4 * 64 + lane;
-                    v[r] = ((b[0] + b[64]) + b[128]) + b[192];
+                    const A *b = lds + (k + r) * kW * 64 + lane;
+                    v[r] = b[0] + b[64];
+#pragma unroll
+                    for (int w = 2; w < kW; w++) v[r] += b[64 * w];
                 }
                 S.e[i][j] = cx(v[0], v[1]);
             }
@@ -613,7 +616,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     M3<A> up, dn;
     {
         M3<A> X, Z;
-        direction_terms<A, GF_MAG, LF>(a, fg, mu, c, par, site_off, X, Z);
+        direction_terms<A, LF>(a, fg, mu, c, par, site_off, X, Z);
         prod_ab(up, Gx, X);    // g(x) U0_mu(x) g(y)^dagger
         prod_adag(dn, Z, Gx);  // g(z) U0_mu(z) g(x)^dagger
     }
@@ -720,7 +723,7 @@ template <typename A, int LF> hipError_t launch_mag_lf(int algo, int theta, cons
 template <typename A, int G, int ALGO, int THETA, int LF>
 void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)sweep_grid(a.geo)), block(kGBlock);
+    const dim3 grid((unsigned)sweep_grid(a.geo)), block(64 * gsweep_waves<G>());
     hipLaunchKernelGGL((k_gsweep<A, G, ALGO, THETA, LF>), grid, block, 0, s, a);
 }
 

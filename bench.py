@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--omega", type=float, default=1.7)
     ap.add_argument("--precision", type=float, default=1e-12)
     ap.add_argument("--switch", type=float, default=1e-12)
-    ap.add_argument("--switch-northstar", type=float, default=1e-6)
+    ap.add_argument("--early-switch", type=float, default=1e-6)
     ap.add_argument("--refresh", type=int, default=50)
     ap.add_argument("--check32", type=int, default=5)
     ap.add_argument("--reproject", type=int, default=100)
@@ -417,8 +417,8 @@ def main():
     ns = None
     modes = {}
     if not args.no_fp64:
-        ns = mixed(args.mode, args.switch_northstar)
-        print(f"bench: mixed, switch {args.switch_northstar:g}: {ns}", file=sys.stderr, flush=True)
+        ns = mixed(args.mode, args.early_switch)
+        print(f"bench: mixed, switch {args.early_switch:g}: {ns}", file=sys.stderr, flush=True)
         for m in (0, 1, 2, 3):
             rm = r if m == args.mode else mixed(m, args.switch)
             modes[MODE_NAMES[m]] = {"time_to_theta_s": rm.ms_total / 1e3, "sweeps": rm.sweeps,

@@ -702,7 +702,7 @@ constexpr int kChunk = 32;  // sweeps enqueued between two host reads of the dev
 
 double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12.; }
 
-// theta below which a single-precision phase is judged for stagnation: the
+// theta below which a single-precision phase is tested for stagnation: the
 // rounding floor of FP32 sweeps lies near 1e-12 for the squared theta of
 // Landau, Coulomb and U1xU1, and higher for MAG's theta, a sum of |X|
 double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
@@ -726,7 +726,7 @@ int check_segments(const int dims[4], int gauge) { return gauge == GF_COULOMB &&
 // coarse_every > check_every: while the last theta the host has read is above
 // kCoarseFactor * tol (or before the first check), check only every
 // coarse_every sweeps.  Such a theta cannot reach tol within the next chunk,
-// and stagnation is judged far below it, so only the cost of the checks
+// and stagnation is tested far below it, so only the cost of the checks
 // changes.  0 checks every check_every sweeps throughout.
 constexpr double kCoarseFactor = 1e3;
 

@@ -17,7 +17,7 @@ struct DevState {
     int checks;           // number of completed checks
     int stall;            // consecutive checks without halving the best theta
     int stall_limit;      // 0: no stagnation exit
-    double stall_below;   // stagnation is only judged for theta below this
+    double stall_below;   // stagnation is only tested for theta below this
     double best;          // smallest theta seen
     double tol;           // theta threshold
     double theta_scale;   // 1 / (Nc * V)

@@ -254,7 +254,7 @@ __device__ __forceinline__ void check_update(DevState *st, double theta, double 
     // stall_below, without lowering the best theta by 10%.  Single-precision
     // sweeps that have reached their rounding floor gain nothing; above
     // stall_below theta may still rise for a while (overrelaxation is not
-    // monotone, and a Coulomb lattice tests its worst slice) and is not judged.
+    // monotone, and a Coulomb lattice tests its worst slice) and is not tested.
     if (theta < 0.9 * st->best) {
         st->best = theta;
         st->stall = 0;

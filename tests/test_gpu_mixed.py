@@ -47,6 +47,36 @@ def test_gauge_sweeps_track_fp64_link_sweeps(gf, gpu, oracle, mode, tol, gauge):
     assert err < tol, err
 
 
+@pytest.mark.parametrize("mode,tol", [(1, 2e-5), (2, 2e-6), (3, 2e-5)])
+def test_coulomb_gauge_sweeps_track_fp64_slice_sweeps(gf, gpu, oracle, mode, tol):
+    """Coulomb on a 4D lattice: three gauge-transformation sweeps of every
+    timeslice at once (blocks of three waves, one per spatial direction),
+    applied to U0 and projected, against three FP64 OR sweeps of each slice
+    by the oracle followed by projectSU3, compared on every slice's spatial
+    links.  The slices are independent (no time term in the Coulomb update);
+    the oracle sweeps slice t in the 4D checkerboard's order, starting with
+    the sites whose 3D parity is t mod 2."""
+    dims = (4, 6, 4, 8)
+    dims3 = (1,) + dims[1:]
+    V3 = int(np.prod(dims[1:]))
+    U = _cfg(gf, dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    res = gf.run_mixed(gf.Lattice(dims), d, COULOMB, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=3,
+                       mode=mode, check32=10)
+    assert res.sweeps == res.sweeps_fp32 == 3
+    S = to_timeslices(U, dims)
+    got = to_timeslices(d.download(), dims)
+    for t in range(dims[0]):
+        up, dw = S[t].reshape(-1).copy(), S[t - 1].reshape(-1).copy()
+        for _ in range(3):
+            for parity in (t % 2, 1 - t % 2):
+                oracle.sweep_coulomb(OR, dims3, up, dw, parity, 1.7)
+        oracle.project_all(dims3, up)
+        spatial = lambda a: a.reshape(4, 18, V3)[1:, :12, :]  # noqa: E731
+        err = np.abs(spatial(got[t]) - spatial(up)).max()
+        assert err < tol, (t, err)
+
+
 @pytest.mark.parametrize("own_streams", [False, True])
 def test_batch_runs_match_single_runs(gf, gpu, own_streams):
     """gf_batch_run_sa and then gf_batch_run_mixed on three copies of an 8^4

@@ -8,7 +8,9 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "su3gf_kernels.h"
 #include "su3gf_error.h"
@@ -136,19 +138,44 @@ struct Scratch {
     template <typename T> T *as() const { return (T *)p; }
 };
 
-// pinned host copy of a driver's device state: one per host thread, kept
-// until the process ends (freeing pinned memory would synchronise the device)
+// Pinned host copies of the drivers' device state, one per host thread.  They
+// are never freed (freeing pinned memory would synchronise the device): a
+// thread that ends puts its copy on a process-wide free list, where the next
+// new thread finds it, so threads that come and go (the batch calls' workers)
+// reuse the same few.  Every copy into one is waited for before the driver
+// returns, so a copy on the list is idle.
+std::mutex g_pinned_mu;
+std::vector<DevState *> g_pinned_free;
+
+struct PinnedSlot {
+    DevState *h = nullptr;
+    ~PinnedSlot()
+    {
+        if (!h) return;
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        g_pinned_free.push_back(h);
+    }
+};
+
 hipError_t pinned_state(DevState **out)
 {
-    static thread_local DevState *h = nullptr;
-    if (!h) {
-        hipError_t e = hipHostMalloc((void **)&h, sizeof(DevState), hipHostMallocDefault);
-        if (e != hipSuccess) {
-            h = nullptr;
-            return e;
+    static thread_local PinnedSlot slot;
+    if (!slot.h) {
+        {
+            std::lock_guard<std::mutex> lock(g_pinned_mu);
+            if (!g_pinned_free.empty()) {
+                slot.h = g_pinned_free.back();
+                g_pinned_free.pop_back();
+            }
+        }
+        if (!slot.h) {
+            DevState *h = nullptr;
+            const hipError_t e = hipHostMalloc((void **)&h, sizeof(DevState), hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            slot.h = h;
         }
     }
-    *out = h;
+    *out = slot.h;
     return hipSuccess;
 }
 

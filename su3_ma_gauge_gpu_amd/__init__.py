@@ -454,6 +454,8 @@ def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: flo
     n, lats, ptrs, strs = _batch_args(lattices, Us, streams, max_concurrent)
     seeds = list(seeds) if seeds is not None else [1] * n
     counters = list(counters) if counters is not None else [0] * n
+    if len(seeds) != n or len(counters) != n:
+        raise ValueError("one seed and one counter per problem")
     cs = (C.c_int * n)(*seeds)
     ctr = (C.c_uint32 * n)(*counters)
     temps = (C.c_int * n)()

@@ -113,6 +113,15 @@ def test_batch_calls_check_every_problem_on_the_host(gf):
     # the caller's streams: one per concurrent problem, none of them null
     short = (C.c_void_p * 4)(0x10, 0x20, None, 0x40)
     assert lib.gf_batch_run_mixed(2, lats_ok, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, short, out) == 2
+    # the Python mirror: one seed and one counter per problem, one stream per
+    # concurrent problem, checked before the library is called
+    two = [gf.Lattice((4, 4, 4, 4))] * 2
+    with pytest.raises(ValueError):
+        gf.batch_run_sa(two, [0x1000, 0x2000], seeds=[1], counters=[0, 0])
+    with pytest.raises(ValueError):
+        gf.batch_run_sa(two, [0x1000, 0x2000], seeds=[1, 2], counters=[0, 0, 0])
+    with pytest.raises(ValueError):
+        gf.batch_run_mixed(two, [0x1000, 0x2000], max_concurrent=4, streams=[0x10, 0x20])
 
 
 def test_stats_rejects_bad_arguments(gf):

@@ -7,13 +7,16 @@ against; nothing in the product imports this module.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
-ORACLE_DIR = ROOT / "oracle"
+# SU3GF_ORACLE_DIR points the tests at another build of the oracle, e.g. a
+# copy with a planted mistake, to see which pins catch it
+ORACLE_DIR = Path(os.environ.get("SU3GF_ORACLE_DIR", ROOT / "oracle"))
 
 LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
@@ -169,7 +172,7 @@ class RefHarness:
     """The reference's own lattice / Random123 templates compiled by
     oracle/Makefile (target ``ref``) into oracle/_ref/libsu3gf_ref.so."""
 
-    PATH = ORACLE_DIR / "_ref" / "libsu3gf_ref.so"
+    PATH = ROOT / "oracle" / "_ref" / "libsu3gf_ref.so"
 
     @classmethod
     def available(cls) -> bool:

@@ -15,6 +15,10 @@
 5. The random update rules (heatbath, random transformation, stochastic
    relaxation) one quaternion at a time: Kolmogorov-Smirnov tests against their
    closed-form sampling densities, with power checks, and exact outcomes.
+6. MAG theta against half the gradient of the MAG functional under site
+   rotations (exact, from three samples of a trigonometric polynomial), a
+   closed form on a cold lattice rotated at one site, and its vanishing at
+   a converged maximum.
 """
 import numpy as np
 import pytest
@@ -631,3 +635,118 @@ def test_sr_update_flip_frequency(oracle, p):
         assert flips == int(p) * n
     else:
         assert abs(flips / n - p) < 5 * np.sqrt(p * (1 - p) / n), flips / n
+
+
+# ------------------------------------------------------------ MAG theta
+# MAGKernelsSU3.hxx:116-223: theta_x = sum_k |X_k| over the subgroups
+# k = (0,1), (0,2), (1,2), with X_k summed from the up links' 2x2 blocks and
+# the down links' Hermitian blocks.  The pins below use only the MAG
+# functional, sum_j |U_jj|^2 over a site's eight links, and rotations: for a
+# gauge rotation exp(i t sigma) of site x in subgroup k (up links rotated from
+# the left, down links from the right by its inverse), the functional is
+# A + B cos 2t + C sin 2t, and X_k is half its gradient at t = 0 over
+# sigma_2 (real part) and sigma_1 (imaginary part).  With that trigonometric
+# form the derivative is exactly F(pi/4) - F(-pi/4): no finite-difference
+# error.  A squared |X_k|, a dropped or non-Hermitian down term, a wrong slot
+# i+j-1 or a factor slip each move theta_x off half the gradient norm.
+MAG_SUBGROUPS = ((0, 1), (0, 2), (1, 2))
+PAULI = {1: np.array([[0, 1], [1, 0]], dtype=complex), 2: np.array([[0, -1j], [1j, 0]])}
+
+
+def _su2_embed(i, j, k, t):
+    R = np.eye(3, dtype=complex)
+    R[np.ix_([i, j], [i, j])] = np.cos(t) * np.eye(2) + 1j * np.sin(t) * PAULI[k]
+    return R
+
+
+def _site_links(oracle, dims, U):
+    """Up links U_mu(x) and down links U_mu(x - mu) of every site: [mu, idx, 3, 3]."""
+    V = int(np.prod(dims))
+    M = to_matrices(U, V)
+    nn = oracle.neighbour_table(dims).reshape(8, V)
+    return M, np.stack([M[mu, nn[2 * mu]] for mu in range(4)])
+
+
+def _mag_local(up, dn):
+    d = np.abs(np.diagonal(up, axis1=-2, axis2=-1)) ** 2 + np.abs(np.diagonal(dn, axis1=-2, axis2=-1)) ** 2
+    return d.sum(axis=(0, 2))
+
+
+def _mag_theta_from_gradient(up, dn):
+    theta = 0.0
+    for i, j in MAG_SUBGROUPS:
+        grad = []
+        for k in (1, 2):
+            fp, fm = [_mag_local(R @ up, dn @ np.conj(R.T)) for R in
+                      (_su2_embed(i, j, k, np.pi / 4), _su2_embed(i, j, k, -np.pi / 4))]
+            grad.append(fp - fm)
+        theta = theta + 0.5 * np.hypot(grad[0], grad[1])
+    return theta
+
+
+@pytest.mark.parametrize("kind", ["hot", "smooth"])
+def test_mag_theta_is_half_the_gauge_gradient(oracle, gf, kind):
+    dims = (4, 6, 4, 8)
+    U = random_lattice(dims, 13) if kind == "hot" else gf.make_config(dims, 0.35, 1, 2)
+    up, dn = _site_links(oracle, dims, U)
+    gff_x, theta_x = oracle.quality_per_site(MAG, dims, U)
+    expect = _mag_theta_from_gradient(up, dn)
+    assert expect.min() > 1e-3  # far from a maximum: every site has a gradient
+    np.testing.assert_allclose(theta_x, expect, rtol=1e-12, atol=1e-13)
+    # gff_x: sum_j |U_mu(x)_jj|^2 over the up links alone (MAGKernelsSU3.hxx:184-186)
+    np.testing.assert_allclose(gff_x, (np.abs(np.diagonal(up, axis1=-2, axis2=-1)) ** 2).sum(axis=(0, 2)),
+                               rtol=1e-14)
+
+
+@pytest.mark.parametrize("sub", MAG_SUBGROUPS)
+@pytest.mark.parametrize("k", [1, 2])
+def test_mag_theta_gauge_rotation_closed_form(oracle, sub, k):
+    """A cold lattice rotated at one site x0 by g = exp(i eps sigma_k) in one
+    subgroup: the eight links of x0 contribute -i sin 2 eps (sigma_1) or
+    -sin 2 eps (sigma_2) each, so theta_x0 = 8 |sin 2 eps|; each of the eight
+    neighbours sees one rotated link, |sin 2 eps|; every other site 0.  The
+    lattice values, with the reference's prefactors 1/Nc and 1/(Nc Ndim)
+    (MAGKernelsSU3.hxx:76-83) and the average over V: theta = 16 sin 2 eps /
+    (3 V), gff = 1 - 4 sin^2 eps / (3 V)."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    eps = 0.3
+    g = _su2_embed(*sub, k, eps)
+    M = np.broadcast_to(np.eye(3, dtype=complex), (4, V, 3, 3)).copy()
+    nn = oracle.neighbour_table(dims).reshape(8, V)
+    x0 = 37
+    for mu in range(4):
+        M[mu, x0] = g
+        M[mu, nn[2 * mu, x0]] = np.conj(g.T)
+    U = from_matrices(M)
+    _, theta_x = oracle.quality_per_site(MAG, dims, U)
+    want = np.zeros(V)
+    want[x0] = 8 * np.sin(2 * eps)
+    for mu in range(4):
+        want[nn[2 * mu, x0]] = np.sin(2 * eps)
+        want[nn[2 * mu + 1, x0]] = np.sin(2 * eps)
+    np.testing.assert_allclose(theta_x, want, atol=1e-15, rtol=1e-14)
+    gff, theta = oracle.quality(MAG, dims, U)
+    assert theta == pytest.approx(16 * np.sin(2 * eps) / (3 * V), rel=1e-14)
+    assert gff == pytest.approx(1 - 4 * np.sin(eps) ** 2 / (3 * V), rel=1e-14)
+
+
+def test_mag_theta_vanishes_at_converged_maximum(oracle, gf):
+    """OR run by the oracle to theta < 1e-14 ends at a maximum of the MAG
+    functional: no site rotation in any subgroup raises it at first order,
+    and every theta_x is at rounding level.  A down term with the wrong sign
+    would not cancel the up terms there."""
+    dims = (2, 4, 2, 4)
+    U = gf.make_config(dims, 0.3, 1, 2)
+    n, g, t = oracle.run_or(MAG, dims, U, 1.7, 1e-14, 5000, 1, 100)
+    assert n < 5000 and t < 1e-14
+    up, dn = _site_links(oracle, dims, U)
+    assert np.abs(_mag_theta_from_gradient(up, dn)).max() < 1e-13
+    _, theta_x = oracle.quality_per_site(MAG, dims, U)
+    assert theta_x.max() < 1e-13
+    base = _mag_local(up, dn)
+    for i, j in MAG_SUBGROUPS:
+        for k in (1, 2):
+            for eps in (1e-3, -1e-3):
+                R = _su2_embed(i, j, k, eps)
+                assert (_mag_local(R @ up, dn @ np.conj(R.T)) - base).max() < 1e-12

@@ -96,7 +96,8 @@ def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, va
     HIP events around nsweeps back-to-back sweeps (2 * nsweeps launches) on
     `stream`.  variant: 0 link sweep in the precision of `lat` with the fused
     check, 1 FP32 links + g, 2 / 3 gauge-transformation sweep in FP32 / FP64
-    arithmetic, 4 the FP32 gauge-transformation sweep on 24-bit links."""
+    arithmetic, 4 the FP32 gauge-transformation sweep on 24-bit links, 5 the
+    drop-in quality kernel (nsweeps launches; the time is per launch)."""
     ms = C.c_float()
     _lib.call("gf_time_sweeps", C.byref(lat.struct), C.c_void_p(U.ptr), int(gauge), int(nsweeps), int(variant),
               C.byref(ms), stream)
@@ -181,6 +182,10 @@ def workload_coulomb(args):
     """Coulomb gauge of a 32^3 x 64 lattice (sigma 0.3, seed 1, Haar seed 2),
     every timeslice at once: mixed (FP32 gauge-transformation sweeps, then
     FP64) against all-FP64, each slice to theta < 1e-12."""
+    print(json.dumps(coulomb_line(args)), flush=True)
+
+
+def coulomb_line(args) -> dict:
     dims = (64, 32, 32, 32)
     V = int(np.prod(dims))
     U0 = gf.make_config(dims, 0.3, 1, 2)
@@ -219,7 +224,7 @@ def workload_coulomb(args):
         "max_slice_gff_rel_diff": float(np.max(np.abs(g_m - g_f) / np.abs(g_f))),
         "gff_mixed": r.gff, "gff_fp64": f.gff,
     })
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def workload_mag(args):
@@ -280,7 +285,7 @@ def workload_mag(args):
     U32 = gf.DeviceBuffer(lat32.nbytes)
     kms, frac = {}, {}
     for var, name, bsu in ((4, "gauge_sweep_q24", 432), (2, "gauge_sweep_f32", 528), (1, "sweep_f32_r12_gauge", 864),
-                           (0, "sweep_f64_pattern18", 1536)):
+                           (0, "sweep_f64_tiled12", 1536)):
         if var:
             _lib.call("gf_convert", C.byref(lat.struct), C.c_void_p(src.ptr), C.byref(lat32.struct),
                       C.c_void_p(U32.ptr), 0, stream)
@@ -295,6 +300,10 @@ def workload_mag(args):
 
 
 def workload_gribov(args):
+    print(json.dumps(gribov_line(args)), flush=True)
+
+
+def gribov_line(args, streams=None) -> dict:
     """Gauge copies: for each of 16^4, 20^4, 24^4, 32^4 (sigma 0.3, seed 1),
     four copies with Haar seeds 10.. (one per copy); each copy is annealed
     (T 1.45 -> 0.01 in args.sa_steps steps, 3 microcanonical steps per
@@ -306,7 +315,8 @@ def workload_gribov(args):
     streams are created here, before the first device allocation of the
     workload (streams created after it have run the concurrent copies up to
     35% slower with the runtime's 4 hardware queues per process)."""
-    streams = [_stream() for _ in range(args.copies)]
+    if streams is None:
+        streams = [_stream() for _ in range(args.copies)]
     sizes = [16, 20, 24, 32]
     total_su, total_s, per_size = 0, 0.0, {}
     haar = 10
@@ -341,7 +351,7 @@ def workload_gribov(args):
                        "sigma": 0.3, "seed": 1, "haar_seeds": [10, haar - 1], "sa": [1.45, 0.01, args.sa_steps, 3],
                        "precision": args.precision, "mode": MODE_NAMES[args.mode]})
     line.update({"total_s": total_s, "per_size": per_size})
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -394,7 +404,7 @@ def main():
     su_per_s = sweeps * V / (ms_total / 1e3)
     print(f"bench: mixed {r}", file=sys.stderr, flush=True)
 
-    # plain FP64 reference run (same driver, FP64 18-real layout throughout)
+    # plain FP64 reference run (same driver, all sweeps FP64, on its tiled copy of the links)
     fp64 = None
     if not args.no_fp64:
         reset()
@@ -439,9 +449,12 @@ def main():
         fresh32()
         kms[name] = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var)
     reset()
-    kms["sweep_f64_pattern18"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 0)
+    # the FP64 phase's kernel, on the drivers' tiled copy of the links
+    kms["sweep_f64_tiled12"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 0)
     reset()
     kms["dropin_sweep_f64_pattern18"] = dropin_kernel_ms(lat64, work, stream, args.kernel_sweeps)
+    # the drop-in quality kernel (GaugeFixingStats::generateGaugeQuality's per-site pass)
+    kms["dropin_quality_f64_pattern18"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 5)
     # algorithmic bytes per parity launch (V/2 site-updates):
     #   link sweeps: 16 R sizeof(real), R = 12 (rows 0,1 of every link read and written once);
     #   FP32 links + g: also the site's 12-float g read and written (+96 B);
@@ -453,15 +466,17 @@ def main():
         "gauge_sweep_f32": (V // 2) * 528,
         "gauge_sweep_f64arith": (V // 2) * 528,
         "gauge_sweep_q24": (V // 2) * 432,
-        "sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
+        "sweep_f64_tiled12": (V // 2) * 16 * 12 * 8,
         "dropin_sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
+        # quality: every link's rows 0,1 read once per launch (V sites x 4 links x 12 reals)
+        "dropin_quality_f64_pattern18": V * 4 * 12 * 8,
     }
     t32 = r.ms_fp32 / max(r.sweeps_fp32, 1)
     t64 = (r.ms_total - r.ms_fp32) / max(r.sweeps - r.sweeps_fp32, 1)
     k32name = {0: "sweep_f32_r12_gauge", 1: "gauge_sweep_f32", 2: "gauge_sweep_f64arith",
                3: "gauge_sweep_q24"}[args.mode]
-    dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms["sweep_f64_pattern18"]
-    kname = k32name if dom32 else "sweep_f64_pattern18"
+    dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms["sweep_f64_tiled12"]
+    kname = k32name if dom32 else "sweep_f64_tiled12"
     achieved = kbytes[kname] / (kms[kname] / 1e3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"

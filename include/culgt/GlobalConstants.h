@@ -96,11 +96,32 @@ inline int sweep_nsb(int a, int b, const lat_coord_t *size)
     return nsb;
 }
 
+// The reference's single-precision build with DPUPDATES=true
+// (gaugefixing/apps/Makefile:46-48) computes the OR, microcanonical and
+// heatbath updates in double (algorithms/OrUpdate.hxx:54-64,
+// MicroUpdate.hxx:45-54, SaUpdate.hxx:54-98): the same macros select the
+// library's double-precision update rules.
+inline int update_rule(int algo)
+{
+    if (sizeof(Real) == sizeof(double)) return algo;
+#ifdef USE_DP_ORUPDATE
+    if (algo == GF_OR) return algo | GF_UPDATE_DP;
+#endif
+#ifdef USE_DP_MICROUPDATE
+    if (algo == GF_MICRO) return algo | GF_UPDATE_DP;
+#endif
+#ifdef USE_DP_SAUPDATE
+    if (algo == GF_SA) return algo | GF_UPDATE_DP;
+#endif
+    return algo;
+}
+
 inline void sweep(const lat_coord_t *size, int gauge, int algo, int a, int b, Real *U, Real *Udw, bool parity,
                   float param, int seed, unsigned counter)
 {
     const gf_lattice l = lattice(size);
-    check(gf_sweep(&l, U, Udw, parity ? 1 : 0, gauge, algo, param, seed, counter, sweep_nsb(a, b, size), nullptr));
+    check(gf_sweep(&l, U, Udw, parity ? 1 : 0, gauge, update_rule(algo), param, seed, counter,
+                   sweep_nsb(a, b, size), nullptr));
 }
 
 inline void quality_per_site(const lat_coord_t *size, int gauge, int a, int b, Real *U, double *dGff, double *dA)

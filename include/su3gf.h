@@ -43,6 +43,14 @@ typedef enum { GF_LANDAU = 0, GF_COULOMB = 1, GF_MAG = 2, GF_U1XU1 = 3 } gf_gaug
 /* gaugefixing/algorithms/: OrUpdate, MicroUpdate, SaUpdate, SrUpdate, RandomUpdate */
 typedef enum { GF_OR = 0, GF_MICRO = 1, GF_SA = 2, GF_SR = 3, GF_RANDOM = 4 } gf_algo;
 
+/* OR'ed into gf_sweep's algo (GF_OR, GF_MICRO or GF_SA) on FP32 links: the
+ * SU(2) update rule is computed in double from the FP32 subgroup sums and
+ * rounded back, as in the reference's single-precision build with
+ * DPUPDATES=true (-DUSE_DP_ORUPDATE / USE_DP_MICROUPDATE / USE_DP_SAUPDATE,
+ * gaugefixing/apps/Makefile:46-48; algorithms/OrUpdate.hxx:54-64,
+ * MicroUpdate.hxx:45-54, SaUpdate.hxx:54-98).  A no-op on FP64 links. */
+#define GF_UPDATE_DP 0x100
+
 typedef enum { GF_F32 = 0, GF_F64 = 1 } gf_dtype;
 
 typedef enum { GF_LAYOUT_R12 = 12, GF_LAYOUT_PATTERN18 = 18 } gf_layout;
@@ -229,10 +237,20 @@ gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float 
  * the parity of t+x+y+z, so an odd timeslice has its odd sites updated
  * first, where the reference's per-slice loop sweeps every slice even-first:
  * the same maximum, reached along a slightly different path.  Every sweep is
- * checked (the reference's checkprecision = 1); out->sweeps is the index + 1 of the first sweep whose
- * end state met the precision and out->gff / out->theta are that state's; the
- * array then holds that state plus one even half-sweep.  projectSU3 after
- * sweep i when i % reproject == 0.  No reference counterpart (the reference
+ * checked (the reference's checkprecision = 1); out->sweeps is the index + 1
+ * of the sweep whose end state met the precision, out->gff / out->theta are
+ * that state's, and the array holds that state, projected (the reference
+ * breaks after the check and projects, LandauGaugeFixingSU3_4D.cu:254-271).
+ * A check fused into the even kernel sees the state only after that kernel
+ * has updated it, so once theta < 2 * precision each check takes the even
+ * sites' theta from a read-only pass before the even update, which does not
+ * run if the check converged.  Should theta fall from above 2 * precision to
+ * below it in one sweep, the run goes on and ends on the next sweep's check
+ * (one sweep more than the reference loop).  projectSU3 after sweep i when
+ * i % reproject == 0, unless sweep i follows convergence.  Not converged:
+ * out->theta / out->gff describe the final state, theta as the check
+ * defines it (for Coulomb the largest timeslice theta).  No reference
+ * counterpart (the reference
  * checks with a separate kernel and a host copy,
  * gaugefixing/GaugeFixingStats.hxx:186-232). */
 gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
@@ -246,8 +264,10 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
  * every check32 sweeps): 1 FP32 links with gauge accumulation, 2 / 3 the
  * gauge-transformation sweep with FP32 / FP64 arithmetic (U is then the
  * read-only link copy), 4 the same as 2 on the links packed into 24-bit
- * fixed point.  U is modified by variants 0 and 1.  Any gauge but Coulomb
- * on a single timeslice. */
+ * fixed point; 5 the drop-in quality kernel (gf_stats_quality's per-site
+ * pass, nsweeps launches, any dtype and layout; the time is per launch).
+ * U is modified by variants 0 and 1.  Any gauge but Coulomb on a single
+ * timeslice. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
 
@@ -259,7 +279,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  * theta by 10%); then
  * U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
- * continue to theta < precision, checked every sweep.
+ * continue to theta < precision, checked every sweep; the array then holds
+ * the state that met it, as in gf_run_fast (with switch_precision <
+ * 2 * precision the FP64 checks are made before the even update from the
+ * first sweep on).
  *   mode GF_MIXED_LINKS: a compact FP32 copy of the links is swept and g
  *     updated alongside; every `refresh` sweeps the FP32 links are rebuilt
  *     as G U0 G^dagger (reprojected every `reproject` sweeps in between).

@@ -28,7 +28,10 @@
  * is compiled with -ffp-contract=off, so every multiply and add rounds as in
  * the reference source text.  CUDA's rsqrt is restated as 1/sqrt; cospi and
  * sinpi are restated with an exact argument reduction (glibc has neither).
- * Build twice: the default is Real = double; -DORC_F32 gives Real = float.
+ * Built three times: the default is Real = double; -DORC_F32 gives Real =
+ * float; -DORC_F32 -DORC_DP_UPDATES the float build whose OR, microcanonical
+ * and heatbath updates are computed in double (DPUPDATES=true,
+ * gaugefixing/apps/Makefile:46-48).
  *
  * Pins (tests/test_oracle.py): golden vectors written from the reference's
  * own headers, Random123 known answers, closed forms and invariants of every
@@ -398,6 +401,42 @@ typedef struct {
     orc_rng *rng;
 } orc_algo;
 
+#if defined(ORC_F32) && defined(ORC_DP_UPDATES)
+/* The single-precision build with DPUPDATES=true (gaugefixing/apps/Makefile:46-48):
+ * the OR, microcanonical and heatbath updates in double on float links.  The
+ * squares of the subgroup sums are float expressions assigned to doubles (shA
+ * is Real = float); every result is rounded to float as it is stored. */
+typedef double SaReal;
+#define SA_SQRT(x) sqrt(x)
+
+/* OrUpdate.hxx:54-64 (USE_DP_ORUPDATE) */
+static void upd_or(Real a[4], float orParameter)
+{
+    double ai_sq = a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+    double a0_sq = a[0] * a[0];
+    double b = ((double)orParameter * a0_sq + ai_sq) / (a0_sq + ai_sq);
+    double c = 1.0 / sqrt(a0_sq + b * b * ai_sq);
+    a[0] *= c;
+    a[1] *= b * c;
+    a[2] *= b * c;
+    a[3] *= b * c;
+}
+
+/* MicroUpdate.hxx:45-54 (USE_DP_MICROUPDATE) */
+static void upd_micro(Real a[4])
+{
+    double ai_sq = a[1] * a[1] + a[2] * a[2] + a[3] * a[3];
+    double a0_sq = a[0] * a[0];
+    double b = 2. * a[0] / (a0_sq + ai_sq);
+    a[0] = (Real)((a0_sq - ai_sq) / (a0_sq + ai_sq));
+    a[1] *= b;
+    a[2] *= b;
+    a[3] *= b;
+}
+#else
+typedef Real SaReal;
+#define SA_SQRT(x) ORC_SQRT(x)
+
 /* OrUpdate.hxx:52-78 */
 static void upd_or(Real a[4], float orParameter)
 {
@@ -422,6 +461,7 @@ static void upd_micro(Real a[4])
     a[2] *= b;
     a[3] *= b;
 }
+#endif
 
 /* SrUpdate.hxx:54-96 */
 static void upd_sr(Real a[4], float srParameter, orc_rng *rng)
@@ -455,36 +495,38 @@ static void upd_random(Real a[4], orc_rng *rng)
     a[3] = sin_alpha * cos_theta;
 }
 
-/* SaUpdate.hxx:52-147 (Kennedy-Pendleton heatbath; temperature = 1/beta) */
+/* SaUpdate.hxx:52-147 (Kennedy-Pendleton heatbath; temperature = 1/beta);
+ * SaReal is double in the DPUPDATES build (SaUpdate.hxx:54-98), where the
+ * generator still returns floats */
 static void upd_sa(Real a[4], float temperature, orc_rng *rng)
 {
-    Real e0, e1, e2, e3, dk, p0;
-    Real r1, r2, r3, r4;
-    Real a0, a1, a2, a3;
-    Real delta, phi, sin_alpha, sin_theta, cos_theta;
+    SaReal e0, e1, e2, e3, dk, p0;
+    SaReal r1, r2, r3, r4;
+    SaReal a0, a1, a2, a3;
+    SaReal delta, phi, sin_alpha, sin_theta, cos_theta;
     e0 = a[0];
     e1 = -a[1];
     e2 = -a[2];
     e3 = -a[3];
-    dk = (Real)(1.0 / ORC_SQRT(e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3));
+    dk = (SaReal)(1.0 / SA_SQRT(e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3));
     p0 = dk * temperature;
     do {
         while ((r1 = rng_rand(rng)) < 0.0001) {}
-        r1 = (Real)(-log(r1)) * p0;
+        r1 = (SaReal)(-log(r1)) * p0;
         while ((r2 = rng_rand(rng)) < 0.0001) {}
-        r2 = (Real)(-log(r2)) * p0;
-        r3 = (Real)orc_cospi(2.0 * rng_rand(rng));
+        r2 = (SaReal)(-log(r2)) * p0;
+        r3 = (SaReal)orc_cospi(2.0 * rng_rand(rng));
         r3 = r3 * r3;
         delta = r2 + r1 * r3;
         r4 = rng_rand(rng);
     } while (r4 * r4 > (1.0 - 0.5 * delta));
-    a0 = (Real)(1.0 - delta);
-    cos_theta = (Real)(2.0 * rng_rand(rng) - 1.0);
-    sin_theta = (Real)sqrt(1.0 - cos_theta * cos_theta);
-    sin_alpha = (Real)sqrt(1 - a0 * a0);
-    phi = (Real)(2.0 * rng_rand(rng));
-    a1 = sin_alpha * sin_theta * (Real)orc_cospi(phi);
-    a2 = sin_alpha * sin_theta * (Real)orc_sinpi(phi);
+    a0 = (SaReal)(1.0 - delta);
+    cos_theta = (SaReal)(2.0 * rng_rand(rng) - 1.0);
+    sin_theta = (SaReal)sqrt(1.0 - cos_theta * cos_theta);
+    sin_alpha = (SaReal)sqrt(1 - a0 * a0);
+    phi = (SaReal)(2.0 * rng_rand(rng));
+    a1 = sin_alpha * sin_theta * (SaReal)orc_cospi(phi);
+    a2 = sin_alpha * sin_theta * (SaReal)orc_sinpi(phi);
     a3 = sin_alpha * cos_theta;
     e0 *= dk;
     e1 *= dk;

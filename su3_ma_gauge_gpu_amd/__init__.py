@@ -45,6 +45,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     SA,
     SR,
     U1XU1,
+    UPDATE_DP,
     FILE_PLAIN,
     FILE_HEADERONLY,
     FILE_VOGT,
@@ -219,9 +220,14 @@ class _GaugeKernels:
     Ndim = 4
     Nc = 3
 
-    def __init__(self, lattice: Lattice, stream=None):
+    def __init__(self, lattice: Lattice, stream=None, dp_updates: bool = False):
+        """dp_updates: on FP32 links, compute the OR, microcanonical and
+        heatbath updates in double, as the reference's single-precision build
+        with DPUPDATES=true (USE_DP_ORUPDATE, USE_DP_MICROUPDATE,
+        USE_DP_SAUPDATE; gaugefixing/apps/Makefile:46-48)."""
         self.lattice = lattice
         self.stream = stream
+        self.dp_updates = dp_updates
 
     def initCacheConfig(self) -> None:
         """No-op: the gfx950 kernels need no cache configuration."""
@@ -237,6 +243,8 @@ class _GaugeKernels:
     def _sweep(self, a, b, U, Udw, parity, algo, param=0.0, seed=0, counter=0):
         lat = self.lattice
         nsb = self._nsb(a, b, lat.V)
+        if self.dp_updates and algo in (OR, MICRO, SA):
+            algo |= UPDATE_DP
         _lib.call("gf_sweep", C.byref(lat.struct), _ptr(U), _ptr(Udw), int(bool(parity)), self.gauge, algo,
                   C.c_float(param), int(seed), C.c_uint32(counter & 0xFFFFFFFF), nsb, _ptr(self.stream))
 

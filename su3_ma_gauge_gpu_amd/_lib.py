@@ -13,6 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libsu3gf.so"
 GF_OK, GF_ERR_BAD_SHAPE, GF_ERR_BAD_ARG, GF_ERR_HIP, GF_ERR_NOT_CONVERGED, GF_ERR_NO_DEVICE, GF_ERR_IO = range(7)
 LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
+UPDATE_DP = 0x100  # OR'ed into an update rule: computed in double on FP32 links
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
 MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24 = 0, 1, 2, 3

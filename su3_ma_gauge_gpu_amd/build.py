@@ -54,6 +54,8 @@ APPS = PKG / "apps"
 APP_VARIANTS = [
     ("gaugefix_x8t8_f64", ["-D_X_=8", "-D_T_=8", "-DDOUBLEPRECISION"]),
     ("gaugefix_x8t8_f32", ["-D_X_=8", "-D_T_=8"]),
+    # the reference's DPUPDATES=true single-precision build (apps/Makefile:46-48)
+    ("gaugefix_x8t8_f32dp", ["-D_X_=8", "-D_T_=8", "-DUSE_DP_ORUPDATE", "-DUSE_DP_MICROUPDATE", "-DUSE_DP_SAUPDATE"]),
     ("gaugefix_x32t32_f64", ["-D_X_=32", "-D_T_=32", "-DDOUBLEPRECISION"]),
 ]
 

@@ -326,7 +326,11 @@ gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int ga
     GF_TRY(check_lattice(lat, &V));
     if (!U) return fail(GF_ERR_BAD_ARG, "U is null");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
+    const bool dp_update = (algo & GF_UPDATE_DP) != 0;
+    algo &= ~GF_UPDATE_DP;
     if (algo < GF_OR || algo > GF_RANDOM) return fail(GF_ERR_BAD_ARG, "unknown update %d", algo);
+    if (dp_update && algo != GF_OR && algo != GF_MICRO && algo != GF_SA)
+        return fail(GF_ERR_BAD_ARG, "double-precision updates exist for OR, microcanonical and heatbath steps");
     if (parity != 0 && parity != 1) return fail(GF_ERR_BAD_ARG, "parity must be 0 or 1, got %d", parity);
     if (gauge == GF_COULOMB) {
         if (lat->dims[0] != 1) return fail(GF_ERR_BAD_SHAPE, "Coulomb sweeps act on a timeslice: dims[0] must be 1");
@@ -359,7 +363,7 @@ gf_status gf_sweep(const gf_lattice *lat, void *U, void *Udw, int parity, int ga
         a.seed = (uint32_t)seed;
         a.counter = counter;
         a.nsb = nsb;
-        e = exact::launch_sweep<float>(lat->layout, gauge, algo, a, S(stream));
+        e = exact::launch_sweep<float>(lat->layout, gauge, dp_update ? algo | kDpUpdate : algo, a, S(stream));
     }
     if (e != hipSuccess) return hip_fail(e, "sweep kernel launch");
     return GF_OK;
@@ -482,10 +486,12 @@ static gf_status quality_launch(const gf_lattice *lat, int gauge, const void *U,
     const Geom g = make_geom(lat->dims, V);
     hipError_t e;
     if (lat->dtype == GF_F64) {
-        QualityArgs<double> a{(const double *)U, g, dGff, dA, partial, crit_max};
+        QualityArgs<double> a{};
+        a.U = (const double *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
         e = exact::launch_quality<double>(lat->layout, gauge, a, s);
     } else {
-        QualityArgs<float> a{(const float *)U, g, dGff, dA, partial, crit_max};
+        QualityArgs<float> a{};
+        a.U = (const float *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
         e = exact::launch_quality<float>(lat->layout, gauge, a, s);
     }
     if (e != hipSuccess) return hip_fail(e, "quality kernel launch");
@@ -738,13 +744,23 @@ double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
 // theta, partial, it)` enqueues one parity kernel of sweep `it`;
 // `after_sweep(it)` runs after sweep `it` (reprojection, refresh).  Sweeps
 // after convergence exit at once.  On return r->theta / r->gff describe the
-// state after the first sweep that met `tol` (the buffer then holds that
-// state plus one even half-sweep), or the final state if maxiter was reached.
+// state after the first sweep that met `tol`, or the final state if maxiter
+// was reached.
 // The end state of sweep `it` is checked when (it + 1) % check_every == 0:
 // its odd kernel forms the odd-site theta, the next even kernel the even-site
 // theta before updating, and a check kernel completes the sum.  The phase ends
 // once theta < tol or, with stall_limit > 0, after stall_limit checks in a row
 // that do not lower the smallest theta seen by 10% (r->converged = 2).
+// exact_end: the buffer must hold the state that met `tol` (the reference
+// breaks right after that check, LandauGaugeFixingSU3_4D.cu:254-271), not that
+// state plus the even half-sweep whose kernel formed the even sites' theta.
+// Once theta falls below kNearFactor * tol the device sets `near`; from then
+// on the even sites' theta comes from a read-only pass (launch theta = 3),
+// checked before the even update, which exits if the check converged.  While
+// the host has not seen theta within kHostNearFactor of tol the passes are not
+// enqueued at all (a theta that falls from above kNearFactor * tol below tol
+// within such a stretch costs one more sweep, whose end state is then the
+// one checked and returned).
 // Coulomb on a 4D lattice: each timeslice is its own problem and must meet
 // the precision on its own.  The per-slice check needs the 64-site blocks to
 // tile the slices; otherwise the lattice average is tested.
@@ -756,34 +772,47 @@ int check_segments(const int dims[4], int gauge) { return gauge == GF_COULOMB &&
 // and stagnation is tested far below it, so only the cost of the checks
 // changes.  0 checks every check_every sweeps throughout.
 constexpr double kCoarseFactor = 1e3;
+constexpr double kNearFactor = 2.0;
+constexpr double kHostNearFactor = 1e3;
 
 template <typename Launch, typename After>
 gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int maxiter, int check_every,
-                     int coarse_every, int stall_limit, int64_t nblocks, FusedCtx &fc, hipStream_t s, Launch launch,
-                     After after_sweep, int *sweeps_done, gf_run_result *r)
+                     int coarse_every, int stall_limit, bool exact_end, bool near, int64_t nblocks, FusedCtx &fc,
+                     hipStream_t s, Launch launch, After after_sweep, int *sweeps_done, gf_run_result *r)
 {
     const int nseg = check_segments(dims, gauge);
     if (check_every < 1) check_every = 1;
     if (nblocks > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
     hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit,
-                                     stall_floor(gauge), s);
+                                     stall_floor(gauge), exact_end ? 1 : 0, kNearFactor * tol, near ? 1 : 0, s);
     if (e != hipSuccess) return hip_fail(e, "state init");
     int it = 0;
     bool done = false;
     bool pending = false;  // a check of the state at the end of sweep it-1 is due
     bool coarse = coarse_every > check_every;
+    bool host_near = exact_end;  // until the first read of this phase's state
+    auto check = [&](int mode, int sweep) -> gf_status {
+        const hipError_t ec = launch_fused_check(fc.st, fc.even(), fc.odd(), nblocks, nseg, sweep, mode, s);
+        if (ec != hipSuccess) return hip_fail(ec, "check kernel launch");
+        return GF_OK;
+    };
     while (it < maxiter && !done) {
         const int every = coarse ? coarse_every : check_every;
         const int n = std::min(kChunk, maxiter - it);
         for (int k = 0; k < n; k++, it++) {
             const bool check_prev = pending;                    // end of sweep it-1
             const bool check_this = (it + 1) % every == 0;      // end of sweep it
+            const bool pre = check_prev && host_near;
+            if (pre) {
+                // read-only theta of the even sites (a no-op until `near`), its
+                // check, then the even update, which exits if that converged
+                e = launch(0, 3, fc.even(), it);
+                if (e != hipSuccess) return hip_fail(e, "theta pass launch");
+                GF_TRY(check(CHECK_PRE, it));
+            }
             e = launch(0, check_prev ? 1 : 0, fc.even(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
-            if (check_prev) {
-                e = launch_fused_check(fc.st, fc.even(), fc.odd(), nblocks, nseg, it, s);
-                if (e != hipSuccess) return hip_fail(e, "check kernel launch");
-            }
+            if (check_prev) GF_TRY(check(pre ? CHECK_FUSED_IF_FAR : CHECK_FUSED, it));
             e = launch(1, check_this ? 2 : 0, fc.odd(), it);
             if (e != hipSuccess) return hip_fail(e, "fused sweep launch");
             pending = check_this;
@@ -792,6 +821,7 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
         GF_TRY(fc.read(s));
         done = fc.host->done != 0;
         if (coarse && fc.host->checks > 0 && !(fc.host->theta > kCoarseFactor * tol)) coarse = false;
+        if (exact_end && fc.host->checks > 0) host_near = fc.host->near || !(fc.host->theta > kHostNearFactor * tol);
     }
     // the host copy of the state is this loop's only after a chunk has run
     // (a phase given no sweeps must not count the previous phase's checks again)
@@ -811,8 +841,8 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
 // the fused loop over the link sweep kernels (FP64, or FP32 with g accumulation)
 template <typename R, typename After>
 gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
-                      int check_every, int coarse_every, int stall_limit, FusedCtx &fc, hipStream_t s,
-                      After after_sweep, int *sweeps_done, gf_run_result *r)
+                      int check_every, int coarse_every, int stall_limit, bool exact_end, bool near, FusedCtx &fc,
+                      hipStream_t s, After after_sweep, int *sweeps_done, gf_run_result *r)
 {
     int64_t V = 1;
     for (int i = 0; i < 4; i++) V *= lat->dims[i];
@@ -830,17 +860,75 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         a.partial = partial;
         return fast::launch_sweep<R>(lat->layout, gauge, GF_OR, theta, gacc != nullptr, a, s);
     };
-    return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit,
+    return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit, exact_end, near,
                       sweep_grid(a.g), fc, s, launch, after_sweep, sweeps_done, r);
 }
 
-gf_status final_quality(const gf_lattice *lat, const void *U, int gauge, gf_run_result *r, hipStream_t s)
+// projectSU3 enqueued by a driver after a sweep: skipped on the device once
+// the run has converged, so that the checked state is what the caller gets.
+// lat may have the drivers' internal layout kTiled12.
+gf_status project_unless_done(const gf_lattice *lat, void *U, const DevState *st, hipStream_t s)
 {
-    gf_stats *st = nullptr;
-    GF_TRY(gf_stats_create(lat->dims, gauge, GF_CRIT_AVERAGE, &st));
-    gf_status rc = gf_stats_quality(st, lat, U, &r->gff, &r->theta, s);
-    gf_stats_destroy(st);
-    return rc;
+    int64_t V = 1;
+    for (int i = 0; i < 4; i++) V *= lat->dims[i];
+    hipError_t e = lat->dtype == GF_F64 ? exact::launch_project<double>(lat->layout, 4, (double *)U, V, s, st)
+                                        : exact::launch_project<float>(lat->layout, 4, (float *)U, V, s, st);
+    if (e != hipSuccess) return hip_fail(e, "project kernel launch");
+    return GF_OK;
+}
+
+// The FP64 phases run on the drivers' own copy of the links in 64-site tiles
+// (kTiled12): a block's 64 sites then read and write one contiguous 6 KB
+// tile per direction instead of twelve 512-byte pieces V * 8 bytes apart.
+struct Tiles64 {
+    Scratch buf;
+    gf_lattice lat{};
+    double *p = nullptr;
+    gf_status init(const gf_lattice *caller, int64_t V, hipStream_t s)
+    {
+        lat = *caller;
+        lat.layout = kTiled12;
+        HIP_TRY(buf.alloc(sizeof(double) * 4 * tiled12_reals(V), s));
+        p = buf.as<double>();
+        return GF_OK;
+    }
+};
+
+// theta and gff of the current state as the fused check defines them (for
+// Coulomb on a 4D lattice the largest timeslice theta): read-only theta
+// passes over both parities and one check, for a run that did not converge
+template <typename R>
+gf_status final_quality_t(const gf_lattice *lat, R *U, int gauge, FusedCtx &fc, gf_run_result *r, hipStream_t s)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    SweepArgs<R> a{};
+    a.U = U;
+    a.Udw = U;
+    a.state = fc.st;
+    a.g = per_slice(make_geom(lat->dims, V), gauge);
+    a.param = 1.7f;
+    a.nsb = 32;
+    const int64_t nb = sweep_grid(a.g);
+    if (nb > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
+    // tol 0 (never met), exact_end and near set: the passes run, the check reads them
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 1, 0.0, 1, s));
+    for (int par = 0; par < 2; par++) {
+        a.parity = par;
+        a.partial = par ? fc.odd() : fc.even();
+        HIP_TRY(fast::launch_sweep<R>(lat->layout, gauge, GF_OR, 3, false, a, s));
+    }
+    HIP_TRY(launch_fused_check(fc.st, fc.even(), fc.odd(), nb, check_segments(lat->dims, gauge), 0, CHECK_PRE, s));
+    GF_TRY(fc.read(s));
+    r->theta = fc.host->theta;
+    r->gff = fc.host->gff;
+    return GF_OK;
+}
+
+gf_status final_quality(const gf_lattice *lat, void *U, int gauge, FusedCtx &fc, gf_run_result *r, hipStream_t s)
+{
+    return lat->dtype == GF_F64 ? final_quality_t<double>(lat, (double *)U, gauge, fc, r, s)
+                                : final_quality_t<float>(lat, (float *)U, gauge, fc, r, s);
 }
 
 }  // namespace
@@ -862,29 +950,38 @@ gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, fl
     hipStream_t s = S(stream);
     FusedCtx fc;
     GF_TRY(fc.init(sweep_grid(per_slice(make_geom(lat->dims, V), gauge)), s));
+    Tiles64 tiles;
+    const bool f64 = lat->dtype == GF_F64;
+    if (f64) GF_TRY(tiles.init(lat, V, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
     HIP_TRY(hipEventRecord(ev.a, s));
+    // FP64: the caller's links into the tiles, swept there, and back
+    if (f64) HIP_TRY(launch_convert(GF_F64, lat->layout, U, GF_F64, kTiled12, tiles.p, V, 0, s));
+    const gf_lattice *wl = f64 ? &tiles.lat : lat;
+    void *W = f64 ? (void *)tiles.p : U;
     auto after = [&](int it) -> gf_status {
-        if (reproject > 0 && it % reproject == 0) return gf_project(lat, U, s);
+        if (reproject > 0 && it % reproject == 0) return project_unless_done(wl, W, fc.st, s);
         return GF_OK;
     };
     int done = 0;
-    gf_status rc = lat->dtype == GF_F64
-                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, fc, s,
-                                             after, &done, out)
-                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, fc, s,
-                                            after, &done, out);
+    gf_status rc = f64 ? fused_phase<double>(wl, (double *)W, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, true,
+                                             false, fc, s, after, &done, out)
+                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, true,
+                                            false, fc, s, after, &done, out);
     GF_TRY(rc);
     out->sweeps = done;
-    GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
+    // third rows (LandauGaugeFixingSU3_4D.cu:271): projectSU3, for FP64 as the
+    // tiles are written back
+    if (f64) HIP_TRY(launch_convert(GF_F64, kTiled12, tiles.p, GF_F64, lat->layout, U, V, 1, s));
+    else GF_TRY(gf_project(lat, U, s));
     HIP_TRY(hipEventRecord(ev.b, s));
     float ms = 0;
     HIP_TRY(hipEventSynchronize(ev.b));
     HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
     out->ms_total = ms;
-    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, out, s));
+    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, fc, out, s));
     return GF_OK;
 }
 
@@ -894,15 +991,40 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
-    if (variant < 0 || variant > 4) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
+    if (variant < 0 || variant > 5) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
     if (gauge == GF_COULOMB && lat->dims[0] == 1) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D lattices");
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
-    if (variant > 0 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
-    if (variant >= 2) GF_TRY(check_gauge_sweep_volume(V));
+    if (variant > 0 && variant < 5 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
+    if (variant >= 2 && variant < 5) GF_TRY(check_gauge_sweep_volume(V));
     hipStream_t s = S(stream);
+    if (variant == 5) {
+        // the drop-in quality kernel (GaugeFixingStats::generateGaugeQuality's
+        // per-site pass with its block partials), U unchanged
+        Scratch part;
+        HIP_TRY(part.alloc(sizeof(double) * 2 * exact::quality_blocks(V), s));
+        EventPair ev;
+        HIP_TRY(hipEventCreate(&ev.a));
+        HIP_TRY(hipEventCreate(&ev.b));
+        const int warm = std::max(2, nsweeps / 2);
+        for (int k = 0; k < warm + nsweeps; k++) {
+            if (k == warm) HIP_TRY(hipEventRecord(ev.a, s));
+            GF_TRY(quality_launch(lat, gauge, U, nullptr, nullptr, part.as<double>(), 0, s));
+        }
+        HIP_TRY(hipEventRecord(ev.b, s));
+        float ms = 0;
+        HIP_TRY(hipEventSynchronize(ev.b));
+        HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+        *ms_per_kernel = ms / nsweeps;
+        return GF_OK;
+    }
     FusedCtx fc;
     GF_TRY(fc.init(sweep_grid(make_geom(lat->dims, V)), s));
+    Tiles64 tiles;  // FP64: the drivers' tiled copy, as gf_run_fast sweeps it
+    if (lat->dtype == GF_F64) {
+        GF_TRY(tiles.init(lat, V, s));
+        HIP_TRY(launch_convert(GF_F64, lat->layout, U, GF_F64, kTiled12, tiles.p, V, 0, s));
+    }
     Scratch gbuf, pbuf;
     struct {
         float *p = nullptr;
@@ -917,7 +1039,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         HIP_TRY(launch_pack24(GF_F32, 12, U, pbuf.as<uint32_t>(), V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
-    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, s));
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 0, 0.0, 0, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -928,9 +1050,9 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
                 double *partial = par ? fc.odd() : fc.even();
                 if (lat->dtype == GF_F64) {
                     SweepArgs<double> a{};
-                    a.U = (double *)U; a.Udw = a.U; a.state = fc.st; a.g = geo; a.param = 1.7f; a.nsb = 32;
+                    a.U = tiles.p; a.Udw = a.U; a.state = fc.st; a.g = geo; a.param = 1.7f; a.nsb = 32;
                     a.sweep = first + it; a.parity = par; a.partial = partial;
-                    HIP_TRY(fast::launch_sweep<double>(lat->layout, gauge, GF_OR, par + 1, false, a, s));
+                    HIP_TRY(fast::launch_sweep<double>(kTiled12, gauge, GF_OR, par + 1, false, a, s));
                 } else if (variant <= 1) {
                     SweepArgs<float> a{};
                     a.U = (float *)U; a.Udw = a.U; a.gacc = gb.p; a.state = fc.st; a.g = geo; a.param = 1.7f;
@@ -997,6 +1119,8 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     } b{sU32.p, sg.p};
     FusedCtx fc;
     GF_TRY(fc.init(sweep_grid(geo), s));
+    Tiles64 tiles;  // the FP64 phase's links
+    GF_TRY(tiles.init(lat, V, s));
     EventPair ev;
     hipEvent_t mid = nullptr;
     HIP_TRY(hipEventCreate(&ev.a));
@@ -1028,8 +1152,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
             return GF_OK;
         };
         GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32,
-                                  kCoarse * check32, kStall32,
-                                  fc, s, after32, &done32, out));
+                                  kCoarse * check32, kStall32, false, false, fc, s, after32, &done32, out));
     } else {
         // the links stay U0 (FP32 copy, read only); only g is swept, and the
         // kernel reprojects it in every update
@@ -1048,25 +1171,30 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         };
         auto none = [](int) -> gf_status { return GF_OK; };
         GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kCoarse * check32, kStall32,
-                          sweep_grid(geo),
-                          fc, s, launch, none, &done32, out));
+                          false, false, sweep_grid(geo), fc, s, launch, none, &done32, out));
     }
     out->sweeps_fp32 = done32;
-    // switch: apply the accumulated transformation to the FP64 configuration
+    // switch: apply the accumulated transformation to the FP64 configuration,
+    // into the FP64 phase's tiles
     HIP_TRY(launch_project_gauge(g, V, s));
-    HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, U, GF_F64, lat->layout, geo, s));
+    HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, tiles.p, GF_F64, kTiled12, geo, s));
     HIP_TRY(hipEventRecord(mid, s));
     auto after64 = [&](int it) -> gf_status {
-        if (reproject > 0 && it % reproject == 0) return gf_project(lat, U, s);
+        if (reproject > 0 && it % reproject == 0) return project_unless_done(&tiles.lat, tiles.p, fc.st, s);
         return GF_OK;
     };
     int done64 = 0;
     out->converged = 0;
     const int left = std::max(0, maxiter - done32);
-    GF_TRY(fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, left, 1, 0, 0, fc, s, after64, &done64,
-                               out));
+    // an FP32 phase that went down to (near) the precision hands over a state
+    // that may meet it after a sweep or two: check it before the even update
+    // from the start
+    const bool near = switch_precision < kNearFactor * precision;
+    GF_TRY(fused_phase<double>(&tiles.lat, tiles.p, nullptr, gauge, omega, precision, left, 1, 0, 0, true, near, fc,
+                               s, after64, &done64, out));
     out->sweeps = done32 + done64;
-    GF_TRY(gf_project(lat, U, s));  // third rows (LandauGaugeFixingSU3_4D.cu:271)
+    // back to the caller's links, projected (third rows, LandauGaugeFixingSU3_4D.cu:271)
+    HIP_TRY(launch_convert(GF_F64, kTiled12, tiles.p, GF_F64, lat->layout, U, V, 1, s));
     HIP_TRY(hipEventRecord(ev.b, s));
     float ms = 0, ms1 = 0;
     HIP_TRY(hipEventSynchronize(ev.b));
@@ -1074,7 +1202,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     HIP_TRY(hipEventElapsedTime(&ms1, ev.a, mid));
     out->ms_total = ms;
     out->ms_fp32 = ms1;
-    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, out, s));
+    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, fc, out, s));
     return GF_OK;
 }
 

@@ -120,9 +120,22 @@ template <typename R> struct LinkRef {
     int64_t V;
 };
 
+// stride = kTiled12: the drivers' internal FP64 layout, rows 0,1 of each
+// direction in tiles of 64 sites -- component c of the link (idx, mu) at
+// base[mu * tiled12_reals(V) + (idx - idx % 64) * 12 + 64 * c + idx % 64] -- so
+// that a block's 64 sites read one contiguous 6 KB tile per direction
+// instead of twelve 512-byte pieces V * 8 bytes apart.
+constexpr int kTiled12 = -12;
+__host__ __device__ inline int64_t tiled12_reals(int64_t V) { return 12 * 64 * ((V + 63) / 64); }
+
 template <typename R> __device__ __forceinline__ LinkRef<R> link_ref(R *base, int64_t V, int stride, int64_t idx, int mu)
 {
     LinkRef<R> l;
+    if (stride == kTiled12) {
+        l.p = base + mu * tiled12_reals(V) + (idx & ~int64_t(63)) * 12 + (idx & 63);
+        l.V = 64;
+        return l;
+    }
     l.p = base + (int64_t)stride * V * mu + idx;
     l.V = V;
     return l;

@@ -26,12 +26,21 @@ static hipError_t launch_t(const SweepArgs<R> &a, hipStream_t s)
 template <typename R, int G, int S>
 static hipError_t launch_a(int algo, const SweepArgs<R> &a, hipStream_t s)
 {
+    if constexpr (sizeof(R) == 4) {  // FP32 links, update rule in double
+        switch (algo) {
+        case GF_OR + kDpUpdate: return launch_t<R, G, GF_OR + kDpUpdate, S>(a, s);
+        case GF_MICRO + kDpUpdate: return launch_t<R, G, GF_MICRO + kDpUpdate, S>(a, s);
+        case GF_SA + kDpUpdate: return launch_t<R, G, GF_SA + kDpUpdate, S>(a, s);
+        default: break;
+        }
+    }
     switch (algo) {
     case GF_OR: return launch_t<R, G, GF_OR, S>(a, s);
     case GF_MICRO: return launch_t<R, G, GF_MICRO, S>(a, s);
     case GF_SA: return launch_t<R, G, GF_SA, S>(a, s);
     case GF_SR: return launch_t<R, G, GF_SR, S>(a, s);
-    default: return launch_t<R, G, GF_RANDOM, S>(a, s);
+    case GF_RANDOM: return launch_t<R, G, GF_RANDOM, S>(a, s);
+    default: return hipErrorInvalidValue;
     }
 }
 
@@ -59,8 +68,9 @@ template hipError_t launch_sweep<double>(int, int, int, const SweepArgs<double> 
 // one thread per link of `nfields` fields (4 link directions, or 1 for a
 // gauge-transformation array)
 template <typename R, int S>
-__global__ __launch_bounds__(kBlock) void k_project(R *U, int64_t V, int nfields)
+__global__ __launch_bounds__(kBlock) void k_project(R *U, int64_t V, int nfields, const DevState *st)
 {
+    if (st && st->done) return;  // a driver's run has converged: the checked state stays as it is
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= nfields * V) return;
     const int mu = (int)(t / V);
@@ -74,15 +84,18 @@ __global__ __launch_bounds__(kBlock) void k_project(R *U, int64_t V, int nfields
 }
 
 template <typename R>
-hipError_t launch_project(int stride, int nfields, R *U, int64_t V, hipStream_t s)
+hipError_t launch_project(int stride, int nfields, R *U, int64_t V, hipStream_t s, const DevState *st)
 {
     const dim3 grid(blocks_for(nfields * V)), block(kBlock);
-    if (stride == 18) hipLaunchKernelGGL((k_project<R, 18>), grid, block, 0, s, U, V, nfields);
-    else hipLaunchKernelGGL((k_project<R, 12>), grid, block, 0, s, U, V, nfields);
+    if (stride == 18) hipLaunchKernelGGL((k_project<R, 18>), grid, block, 0, s, U, V, nfields, st);
+    else if (stride == kTiled12 && sizeof(R) == 8)
+        hipLaunchKernelGGL((k_project<R, kTiled12>), grid, block, 0, s, U, V, nfields, st);
+    else if (stride == 12) hipLaunchKernelGGL((k_project<R, 12>), grid, block, 0, s, U, V, nfields, st);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
-template hipError_t launch_project<float>(int, int, float *, int64_t, hipStream_t);
-template hipError_t launch_project<double>(int, int, double *, int64_t, hipStream_t);
+template hipError_t launch_project<float>(int, int, float *, int64_t, hipStream_t, const DevState *);
+template hipError_t launch_project<double>(int, int, double *, int64_t, hipStream_t, const DevState *);
 
 // ----------------------------------------------------------------- set hot
 // per site: identity times three random SU(2) subgroup elements from the right,
@@ -131,7 +144,16 @@ template hipError_t launch_set_hot<float>(float *, int64_t, uint32_t, uint32_t, 
 template hipError_t launch_set_hot<double>(double *, int64_t, uint32_t, uint32_t, hipStream_t);
 
 // ----------------------------------------------------------------- quality
-// One thread per site, in the reference's order of operations.
+// One thread per site, in the reference's order of operations
+// (LandauKernelsSU3.hxx:114-191, CoulombKernelsSU3.hxx:110-183,
+// MAGKernelsSU3.hxx:116-223): the per-site values are those of the reference
+// bit for bit.  A block of 256 threads takes the 128 even and the 128 odd
+// sites of parity-split index [128 L, 128 L + 128), i.e. 256 consecutive
+// sites of the lexicographic order, so that most down links a site reads
+// (its z and y neighbours') are the up links its own block reads, and the
+// blocks are dealt out to the eight XCDs in contiguous ranges (t innermost
+// where the timeslices allow), so that the x and t neighbours' links are
+// still in that XCD's L2: each link comes from HBM about once.
 template <typename R, int S>
 __device__ __forceinline__ void load_site_link(const QualityArgs<R> &a, int64_t idx, int mu, M3<R> &u)
 {
@@ -147,6 +169,7 @@ __device__ __forceinline__ void quality_site(const QualityArgs<R> &a, int64_t si
     if (G == GF_MAG) {
         double gff = 0.0;
         double Xx[3] = {0, 0, 0}, Xy[3] = {0, 0, 0};
+#pragma unroll 1
         for (int mu = 0; mu < 4; mu++) {
             M3<R> up, dw;
             load_site_link<R, S>(a, site, mu, up);
@@ -179,6 +202,14 @@ __device__ __forceinline__ void quality_site(const QualityArgs<R> &a, int64_t si
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) sum.e[i][j] = cx((R)0, (R)0);
+    // gff: Re tr of the up links, each trace summed from 0 in the order
+    // (0,0), (1,1), (2,2), directions in order -- the reference's second loop
+    // over the same links (LandauKernelsSU3.hxx:176-188), formed here from
+    // the first loop's loads
+    double result = 0;
+    // one direction's two links at a time (140 VGPRs in FP64, 3 waves per
+    // SIMD); unrolled, all eight links' loads are hoisted (256 VGPRs, 1 wave)
+#pragma unroll 1
     for (int mu = mu0; mu < 4; mu++) {
         M3<R> t;
         load_site_link<R, S>(a, site, mu, t);
@@ -186,6 +217,11 @@ __device__ __forceinline__ void quality_site(const QualityArgs<R> &a, int64_t si
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) sum.e[i][j] = sum.e[i][j] + t.e[i][j];
+        Cx<R> tt = cx((R)0, (R)0);
+        tt = tt + t.e[0][0];
+        tt = tt + t.e[1][1];
+        tt = tt + t.e[2][2];
+        result += (double)tt.x;
         load_site_link<R, S>(a, backward(a.g, c, par, mu), mu, t);
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -214,32 +250,44 @@ __device__ __forceinline__ void quality_site(const QualityArgs<R> &a, int64_t si
             prec += (double)abs2(d);
         }
     dA = prec;
-    double result = 0;
-    for (int mu = mu0; mu < 4; mu++) {
-        M3<R> t;
-        load_site_link<R, S>(a, site, mu, t);
-        Cx<R> tt = cx((R)0, (R)0);
-        tt = tt + t.e[0][0];
-        tt = tt + t.e[1][1];
-        tt = tt + t.e[2][2];
-        result += (double)tt.x;
-    }
     dGff = result;
 }
+
+// logical block of this block, or -1 for a block past the last one
+template <typename R> __device__ __forceinline__ int64_t quality_block(const QualityArgs<R> &a)
+{
+    const uint32_t b = blockIdx.x;
+    if (a.remap == 0) return b;
+    const uint32_t xcd = b & 7, k = b >> 3;
+    if (a.remap == 1) {
+        const int64_t per = (a.nblocks + 7) / 8;
+        const int64_t L = (int64_t)xcd * per + k;
+        return L < a.nblocks ? L : -1;
+    }
+    const uint32_t q = fdiv(k, a.spx_div);
+    const int64_t t = (int64_t)xcd * a.slices_per_xcd + (k - q * (uint32_t)a.slices_per_xcd);
+    return t * a.slice_blocks + q;
+}
+
+constexpr int kQualitySites = 128;  // sites of each parity per block
 
 template <typename R, int G, int S>
 __global__ __launch_bounds__(kBlock) void k_quality(QualityArgs<R> a)
 {
-    const int64_t site = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t L = quality_block(a);
+    const int par = threadIdx.x >= kQualitySites;
+    const int64_t p = L * kQualitySites + (threadIdx.x & (kQualitySites - 1));
+    const bool active = L >= 0 && p < a.g.Vh;
     double th = 0, gf = 0;
-    if (site < a.g.V) {
+    if (active) {
+        const int64_t site = p + (par ? a.g.Vh : 0);
         quality_site<R, G, S>(a, site, th, gf);
         if (a.dGff) {
             a.dGff[site] = gf;
             a.dA[site] = th;
         }
     }
-    if (!a.partial) return;
+    if (!a.partial || L < 0) return;  // both uniform over the block
     __shared__ double sh[2][kBlock / 64];
     double ts = th, gs = gf;
 #pragma unroll
@@ -260,21 +308,34 @@ __global__ __launch_bounds__(kBlock) void k_quality(QualityArgs<R> a)
             T = a.crit_max ? fmax(T, sh[0][k]) : T + sh[0][k];
             Gs += sh[1][k];
         }
-        a.partial[2 * blockIdx.x] = T;
-        a.partial[2 * blockIdx.x + 1] = Gs;
+        a.partial[2 * L] = T;
+        a.partial[2 * L + 1] = Gs;
     }
 }
 
-int64_t quality_blocks(int64_t V) { return (int64_t)blocks_for(V); }
+int64_t quality_blocks(int64_t V) { return (V / 2 + kQualitySites - 1) / kQualitySites; }
 
 template <typename R, int S>
-static hipError_t launch_quality_s(int gauge, const QualityArgs<R> &a, hipStream_t s)
+static hipError_t launch_quality_s(int gauge, QualityArgs<R> a, hipStream_t s)
 {
-    const dim3 grid(blocks_for(a.g.V)), block(kBlock);
+    static_assert(2 * kQualitySites == kBlock, "a block holds both parities' sites");
+    a.nblocks = quality_blocks(a.g.V);
+    const int64_t half_slice = a.g.Vh / a.g.N[0];
+    unsigned grid;
+    if (a.g.N[0] % 8 == 0 && half_slice % kQualitySites == 0 && a.nblocks % 8 == 0) {
+        a.remap = 2;
+        a.slice_blocks = half_slice / kQualitySites;
+        a.slices_per_xcd = a.g.N[0] / 8;
+        a.spx_div = make_fastdiv((uint32_t)a.slices_per_xcd);
+        grid = (unsigned)a.nblocks;
+    } else {
+        a.remap = 1;
+        grid = (unsigned)(8 * ((a.nblocks + 7) / 8));
+    }
     switch (gauge) {
-    case GF_COULOMB: hipLaunchKernelGGL((k_quality<R, GF_COULOMB, S>), grid, block, 0, s, a); break;
-    case GF_MAG: hipLaunchKernelGGL((k_quality<R, GF_MAG, S>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_quality<R, GF_LANDAU, S>), grid, block, 0, s, a); break;
+    case GF_COULOMB: hipLaunchKernelGGL((k_quality<R, GF_COULOMB, S>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case GF_MAG: hipLaunchKernelGGL((k_quality<R, GF_MAG, S>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_quality<R, GF_LANDAU, S>), dim3(grid), dim3(kBlock), 0, s, a); break;
     }
     return hipGetLastError();
 }

@@ -22,6 +22,14 @@ static hipError_t launch_theta(int theta, const SweepArgs<R> &a, hipStream_t s)
         switch (theta) {
         case 1: return launch_t<R, G, A, S, 1, GA>(a, s);
         case 2: return launch_t<R, G, A, S, 2, GA>(a, s);
+        case 3:
+            // the read-only pass: one instantiation per gauge and layout suffices
+            if constexpr (A == GF_OR && !GA) {
+                if (!a.state) return hipErrorInvalidValue;
+                return launch_t<R, G, A, S, 3, false>(a, s);
+            } else {
+                return hipErrorInvalidValue;
+            }
         default: return launch_t<R, G, A, S, 0, GA>(a, s);
         }
     }
@@ -61,6 +69,7 @@ hipError_t launch_sweep<double>(int stride, int gauge, int algo, int theta, bool
                                 hipStream_t s)
 {
     if (gacc) return hipErrorInvalidValue;  // the FP64 phase works on the transformed links directly
+    if (stride == kTiled12) return launch_g<double, kTiled12, false>(gauge, algo, theta, a, s);
     if (stride == 18) return launch_g<double, 18, false>(gauge, algo, theta, a, s);
     return launch_g<double, 12, false>(gauge, algo, theta, a, s);
 }

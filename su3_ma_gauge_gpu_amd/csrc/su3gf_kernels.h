@@ -24,6 +24,21 @@ struct DevState {
     double gff_scale;     // 1 / (Nc * Ndim * V)
     double theta;         // theta of sweep `checked_sweep` (prefactors applied)
     double gff;           // gff of the same state
+    // A phase that must end on the state its last check measured (the FP64
+    // phases, exact_end = 1) cannot stop on a theta formed by an even kernel
+    // that has already updated.  Once theta < near_below (a small multiple of
+    // tol), `near` is set: each check is then made by a read-only theta pass
+    // before the even update, which the check can still cancel.
+    int exact_end;
+    int near;
+    double near_below;
+};
+
+// k_fused_check modes: which theta the check reads
+enum CheckMode {
+    CHECK_FUSED = 0,    // the even sweep kernel's (already updated); acts unless done
+    CHECK_PRE = 1,      // the read-only pass before the even update; acts only if `near`
+    CHECK_FUSED_IF_FAR = 2,  // as CHECK_FUSED, but only while `near` is not set
 };
 
 template <typename R> struct SweepArgs {
@@ -46,6 +61,14 @@ template <typename R> struct QualityArgs {
     double *dGff, *dA;  // optional per-site outputs (V doubles each)
     double *partial;    // optional per-block (theta, gff) partials
     int crit_max;
+    // block order, set by the launcher: a block takes the sites of both
+    // parities with parity-split index in [128 L, 128 L + 128) (256
+    // consecutive sites of the lexicographic order) for its logical block L
+    int remap;              // 0 plain, 1 one contiguous eighth per XCD, 2 the same with t innermost
+    int64_t nblocks;        // logical blocks (quality_blocks)
+    int64_t slice_blocks;   // remap 2: logical blocks per timeslice
+    int slices_per_xcd;     // remap 2
+    FastDiv spx_div;        // division by slices_per_xcd
 };
 
 
@@ -79,10 +102,18 @@ hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, 
 // the 24-bit rows 0,1 -> FP32 rows 0,1 (stride 12), decoded as the sweep kernels decode them
 hipError_t launch_unpack24(const uint32_t *P, float *U32, int64_t V, hipStream_t s);
 
+// added to an update rule's number: the rule computed in double on FP32
+// links (GF_UPDATE_DP); template argument of the exact sweep kernels
+constexpr int kDpUpdate = 8;
+
 // exact translation unit (reference order, no FMA contraction)
 namespace exact {
+// algo: a gf_algo, or GF_OR / GF_MICRO / GF_SA + kDpUpdate (R = float)
 template <typename R> hipError_t launch_sweep(int stride, int gauge, int algo, const SweepArgs<R> &a, hipStream_t s);
-template <typename R> hipError_t launch_project(int stride, int nfields, R *U, int64_t V, hipStream_t s);
+// st: if given, the kernel does nothing once st->done is set (a driver's
+// reprojection enqueued after the sweep that converged)
+template <typename R>
+hipError_t launch_project(int stride, int nfields, R *U, int64_t V, hipStream_t s, const DevState *st = nullptr);
 template <typename R> hipError_t launch_set_hot(R *U, int64_t V, uint32_t seed, uint32_t counter, hipStream_t s);
 template <typename R> hipError_t launch_quality(int stride, int gauge, const QualityArgs<R> &a, hipStream_t s);
 int64_t quality_blocks(int64_t V);
@@ -90,7 +121,8 @@ int64_t quality_blocks(int64_t V);
 
 // fast translation unit (contracted, fused theta, gauge accumulation)
 namespace fast {
-// theta: 0 none, 1 pre-update (even kernel), 2 post-update (odd kernel)
+// theta: 0 none, 1 pre-update (even kernel), 2 post-update (odd kernel),
+// 3 read-only theta pass (OR only; runs only while state->near is set)
 template <typename R>
 hipError_t launch_sweep(int stride, int gauge, int algo, int theta, bool gacc, const SweepArgs<R> &a, hipStream_t s);
 }  // namespace fast
@@ -106,12 +138,12 @@ hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s);
 hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *out, int to, int so, const Geom &geo,
                               hipStream_t s);
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below, hipStream_t s);
+                             double stall_below, int exact_end, double near_below, int near, hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep
 // `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update);
 // nseg > 1: the blocks form nseg independent problems (timeslices), theta is
-// the largest of theirs
+// the largest of theirs; mode: a CheckMode
 hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int nseg,
-                              int sweep, hipStream_t s);
+                              int sweep, int mode, hipStream_t s);
 
 }  // namespace su3gf

@@ -35,6 +35,23 @@ template <typename RI, typename RO>
 static hipError_t launch_convert_t(int si, int so, const RI *src, RO *dst, int64_t V, int reun, hipStream_t s)
 {
     const dim3 grid(blocks_for(4 * V)), block(kBlock);
+    if (si == kTiled12 || so == kTiled12) {
+        // the drivers' FP64 tiles, to and from the caller's FP64 layouts
+        if constexpr (sizeof(RI) == 8 && sizeof(RO) == 8) {
+            if (si == 18 && so == kTiled12)
+                hipLaunchKernelGGL((k_convert<RI, 18, RO, kTiled12>), grid, block, 0, s, src, dst, V, reun);
+            else if (si == 12 && so == kTiled12)
+                hipLaunchKernelGGL((k_convert<RI, 12, RO, kTiled12>), grid, block, 0, s, src, dst, V, reun);
+            else if (si == kTiled12 && so == 18)
+                hipLaunchKernelGGL((k_convert<RI, kTiled12, RO, 18>), grid, block, 0, s, src, dst, V, reun);
+            else if (si == kTiled12 && so == 12)
+                hipLaunchKernelGGL((k_convert<RI, kTiled12, RO, 12>), grid, block, 0, s, src, dst, V, reun);
+            else
+                return hipErrorInvalidValue;
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     if (si == 18 && so == 18) hipLaunchKernelGGL((k_convert<RI, 18, RO, 18>), grid, block, 0, s, src, dst, V, reun);
     else if (si == 18 && so == 12) hipLaunchKernelGGL((k_convert<RI, 18, RO, 12>), grid, block, 0, s, src, dst, V, reun);
     else if (si == 12 && so == 18) hipLaunchKernelGGL((k_convert<RI, 12, RO, 18>), grid, block, 0, s, src, dst, V, reun);
@@ -192,6 +209,11 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
     const dim3 grid(blocks_for(4 * geo.V)), block(kBlock);
     if (to == GF_F64) {
         double *o = (double *)out;
+        if (so == kTiled12) {  // into the drivers' FP64 tiles
+            if (s0 == 18) hipLaunchKernelGGL((k_gauge_apply<18, double, kTiled12>), grid, block, 0, s, U0, g, o, geo);
+            else hipLaunchKernelGGL((k_gauge_apply<12, double, kTiled12>), grid, block, 0, s, U0, g, o, geo);
+            return hipGetLastError();
+        }
         if (s0 == 18 && so == 18) hipLaunchKernelGGL((k_gauge_apply<18, double, 18>), grid, block, 0, s, U0, g, o, geo);
         else if (s0 == 18 && so == 12) hipLaunchKernelGGL((k_gauge_apply<18, double, 12>), grid, block, 0, s, U0, g, o, geo);
         else if (s0 == 12 && so == 18) hipLaunchKernelGGL((k_gauge_apply<12, double, 18>), grid, block, 0, s, U0, g, o, geo);
@@ -207,7 +229,7 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
 
 // ------------------------------------------------------------ driver state
 __global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below)
+                             double stall_below, int exact_end, double near_below, int near)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     st->done = 0;
@@ -223,13 +245,16 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
     st->gff_scale = gff_scale;
     st->theta = 0;
     st->gff = 0;
+    st->exact_end = exact_end;
+    st->near = exact_end && near;
+    st->near_below = near_below;
 }
 
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below, hipStream_t s)
+                             double stall_below, int exact_end, double near_below, int near, hipStream_t s)
 {
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale, stall_limit,
-                       stall_below);
+                       stall_below, exact_end, near_below, near);
     return hipGetLastError();
 }
 
@@ -244,7 +269,11 @@ hipError_t launch_state_init(DevState *st, double tol, double theta_scale, doubl
 // reference fixes them one after another -- and gff the average.
 constexpr int kMaxSeg = 4096;
 
-__device__ __forceinline__ void check_update(DevState *st, double theta, double gff, int sweep)
+// updated: the even kernel that formed the even sites' theta has already
+// applied its update (CHECK_FUSED*), so with exact_end a theta below tol does
+// not end the phase (the state it describes is gone); it sets `near`, and the
+// next check, made before its even update, decides
+__device__ __forceinline__ void check_update(DevState *st, double theta, double gff, int sweep, bool updated)
 {
     st->theta = theta;
     st->gff = gff;
@@ -263,19 +292,23 @@ __device__ __forceinline__ void check_update(DevState *st, double theta, double 
     } else {
         st->stall = 0;
     }
-    if (theta < st->tol) {
+    if (theta < st->tol && !(updated && st->exact_end)) {
         st->done = 1;
         st->converged_sweep = sweep - 1;
     } else if (st->stall_limit > 0 && st->stall >= st->stall_limit) {
         st->done = 2;
         st->converged_sweep = sweep - 1;
     }
+    if (st->exact_end && theta < st->near_below) st->near = 1;
 }
 
 __global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double *even, const double *odd,
-                                                      int64_t nblocks, int nseg, int sweep)
+                                                      int64_t nblocks, int nseg, int sweep, int mode)
 {
     if (st->done) return;
+    if (mode == CHECK_PRE && !st->near) return;
+    if (mode == CHECK_FUSED_IF_FAR && st->near) return;
+    const bool updated = mode != CHECK_PRE;
     __shared__ double sa[1024], sb[1024], sc[1024];
     if (nseg <= 1) {
         double te = 0, ge = 0, to = 0;
@@ -296,7 +329,8 @@ __global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) check_update(st, (sa[0] + sc[0]) * st->theta_scale, sb[0] * st->gff_scale, sweep);
+        if (threadIdx.x == 0)
+            check_update(st, (sa[0] + sc[0]) * st->theta_scale, sb[0] * st->gff_scale, sweep, updated);
         return;
     }
     __shared__ double seg_theta[kMaxSeg];
@@ -327,15 +361,16 @@ __global__ __launch_bounds__(1024) void k_fused_check(DevState *st, const double
             tmax = fmax(tmax, seg_theta[g]);
             ftot += seg_gff[g];
         }
-        check_update(st, tmax * st->theta_scale * nseg, ftot * st->gff_scale, sweep);
+        check_update(st, tmax * st->theta_scale * nseg, ftot * st->gff_scale, sweep, updated);
     }
 }
 
 hipError_t launch_fused_check(DevState *st, const double *even, const double *odd, int64_t nblocks, int nseg,
-                              int sweep, hipStream_t s)
+                              int sweep, int mode, hipStream_t s)
 {
     if (nseg < 1 || nseg > kMaxSeg || nblocks % nseg) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fused_check, dim3(1), dim3(1024), 0, s, st, even, odd, nblocks, nseg, sweep);
+    if (mode < CHECK_FUSED || mode > CHECK_FUSED_IF_FAR) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fused_check, dim3(1), dim3(1024), 0, s, st, even, odd, nblocks, nseg, sweep, mode);
     return hipGetLastError();
 }
 

@@ -150,30 +150,40 @@ template <typename R> __device__ __forceinline__ void upd_random(R a[4], Philox 
     a[3] = sin_alpha * cos_theta;
 }
 
-// SaUpdate.hxx:52-147 (Kennedy-Pendleton heatbath, temperature = 1/beta)
-template <bool E, typename R> __device__ __forceinline__ void upd_sa(R a[4], float temperature, Philox &rng)
+// SaUpdate.hxx:52-147 (Kennedy-Pendleton heatbath, temperature = 1/beta).
+// The reference writes its constants as double literals (0.0001, 2.0, 1.0,
+// 0.5), so with Real = float the exact kernels evaluate those expressions in
+// double as it does -- the acceptance test, cospi(2.0*rand), 1.0-delta,
+// 2.0*rand-1.0 and sqrt(1.0-cos^2) -- with float operands rounded first
+// (r4*r4, cos_theta*cos_theta) and float results; W is that wider type.
+// The fast kernels keep everything in R.
+template <bool E, typename R, typename RAND = R>
+__device__ __forceinline__ void upd_sa(R a[4], float temperature, Philox &rng)
 {
+    using W = std::conditional_t<E, double, R>;
     R e0 = a[0], e1 = -a[1], e2 = -a[2], e3 = -a[3];
     const R dk = inv_sqrt<E>(e0 * e0 + e1 * e1 + e2 * e2 + e3 * e3);
     const R p0 = dk * (R)temperature;
     R delta;
     for (;;) {
         R r1, r2;
-        do { r1 = philox_rand(rng, R(0)); } while (r1 < (R)0.0001);
+        do { r1 = (R)philox_rand(rng, RAND(0)); } while ((W)r1 < (W)0.0001);
         r1 = -log_(r1) * p0;
-        do { r2 = philox_rand(rng, R(0)); } while (r2 < (R)0.0001);
+        do { r2 = (R)philox_rand(rng, RAND(0)); } while ((W)r2 < (W)0.0001);
         r2 = -log_(r2) * p0;
-        R r3 = cospi_((R)2 * philox_rand(rng, R(0)));
+        R r3 = (R)cospi_((W)2 * (W)(R)philox_rand(rng, RAND(0)));
         r3 = r3 * r3;
         delta = r2 + r1 * r3;
-        const R r4 = philox_rand(rng, R(0));
-        if (!(r4 * r4 > (R)1 - (R)0.5 * delta)) break;
+        const R r4 = (R)philox_rand(rng, RAND(0));
+        const R r4sq = r4 * r4;
+        if (!((W)r4sq > (W)1 - (W)0.5 * (W)delta)) break;
     }
-    const R b0 = (R)1 - delta;
-    const R cos_theta = (R)2 * philox_rand(rng, R(0)) - (R)1;
-    const R sin_theta = sqrt_((R)1 - cos_theta * cos_theta);
+    const R b0 = (R)((W)1 - (W)delta);
+    const R cos_theta = (R)((W)2 * (W)(R)philox_rand(rng, RAND(0)) - (W)1);
+    const R cos_sq = cos_theta * cos_theta;
+    const R sin_theta = (R)sqrt_((W)1 - (W)cos_sq);
     const R sin_alpha = sqrt_((R)1 - b0 * b0);
-    const R phi = (R)2 * philox_rand(rng, R(0));
+    const R phi = (R)((W)2 * (W)(R)philox_rand(rng, RAND(0)));
     const R b1 = sin_alpha * sin_theta * cospi_(phi);
     const R b2 = sin_alpha * sin_theta * sinpi_(phi);
     const R b3 = sin_alpha * cos_theta;
@@ -187,13 +197,60 @@ template <bool E, typename R> __device__ __forceinline__ void upd_sa(R a[4], flo
     a[1] = b2 * e3 + b1 * e0 - b3 * e2 - e1 * b0;
 }
 
+// The reference's double-precision updates on FP32 links (USE_DP_ORUPDATE,
+// USE_DP_MICROUPDATE, USE_DP_SAUPDATE: OrUpdate.hxx:54-64, MicroUpdate.hxx:45-54,
+// SaUpdate.hxx:54-98).  The subgroup sums stay float: the squares are float
+// expressions assigned to doubles, as the reference's shA is Real; the rest
+// is double, and each result is rounded to float as it is stored.
+template <bool E> __device__ __forceinline__ void upd_or_dp(float a[4], float omega)
+{
+    const double ai = (double)(a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+    const double a0 = (double)(a[0] * a[0]);
+    const double b = ((double)omega * a0 + ai) / (a0 + ai);
+    const double c = inv_sqrt<E>(a0 + b * b * ai);
+    a[0] = (float)((double)a[0] * c);
+    a[1] = (float)((double)a[1] * (b * c));
+    a[2] = (float)((double)a[2] * (b * c));
+    a[3] = (float)((double)a[3] * (b * c));
+}
+
+__device__ __forceinline__ void upd_micro_dp(float a[4])
+{
+    const double ai = (double)(a[1] * a[1] + a[2] * a[2] + a[3] * a[3]);
+    const double a0 = (double)(a[0] * a[0]);
+    const double b = 2. * (double)a[0] / (a0 + ai);
+    a[0] = (float)((a0 - ai) / (a0 + ai));
+    a[1] = (float)((double)a[1] * b);
+    a[2] = (float)((double)a[2] * b);
+    a[3] = (float)((double)a[3] * b);
+}
+
+// the heatbath in double; the generator still hands out float uniforms
+// (PhiloxWrapper<Real>::rand with Real = float)
+template <bool E> __device__ __forceinline__ void upd_sa_dp(float a[4], float temperature, Philox &rng)
+{
+    double d[4] = {(double)a[0], (double)a[1], (double)a[2], (double)a[3]};
+    upd_sa<E, double, float>(d, temperature, rng);
+#pragma unroll
+    for (int k = 0; k < 4; k++) a[k] = (float)d[k];
+}
+
+// A: a gf_algo, or GF_OR / GF_MICRO / GF_SA + kDpUpdate (FP32 links only)
 template <typename R, int A, bool E> __device__ __forceinline__ void calc_update(R a[4], float param, Philox &rng)
 {
-    if (A == GF_OR) upd_or<E>(a, param);
-    else if (A == GF_MICRO) upd_micro<E>(a);
-    else if (A == GF_SA) upd_sa<E>(a, param, rng);
-    else if (A == GF_SR) upd_sr<E>(a, param, rng);
-    else upd_random(a, rng);
+    if constexpr (A >= kDpUpdate) {
+        static_assert(sizeof(R) == 4, "double-precision updates are for FP32 links");
+        constexpr int B = A - kDpUpdate;
+        if constexpr (B == GF_OR) upd_or_dp<E>(a, param);
+        else if constexpr (B == GF_MICRO) upd_micro_dp(a);
+        else upd_sa_dp<E>(a, param, rng);
+    } else {
+        if (A == GF_OR) upd_or<E>(a, param);
+        else if (A == GF_MICRO) upd_micro<E>(a);
+        else if (A == GF_SA) upd_sa<E>(a, param, rng);
+        else if (A == GF_SR) upd_sr<E>(a, param, rng);
+        else upd_random(a, rng);
+    }
 }
 
 // ---------------------------------------------------- per-link contributions
@@ -367,8 +424,10 @@ __device__ __forceinline__ double wave64_sum(double v)
 
 // ------------------------------------------------------------------- kernel
 // THETA: 0 none, 1 theta of the even sites before the update (even kernel),
-// 2 theta of the odd sites after the update (odd kernel); each block writes its
-// (theta, gff) sums to a.partial[2*block .. 2*block+1].
+// 2 theta of the odd sites after the update (odd kernel), 3 theta of the
+// parity's sites and no update (a read-only pass, run only while
+// a.state->near is set); each block writes its (theta, gff) sums to
+// a.partial[2*block .. 2*block+1].
 // GACC: also left-multiply the site's gauge transformation g(x) (FP32 rows
 // 0,1 in 64-site tiles at a.gacc) by the same SU(2) elements.
 // EXACT exchanges the eight links' contributions and adds them in the
@@ -387,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     R *const tlds = lds + 2 * kQ;
 
     if (a.state && a.state->done) return;  // converged: uniform early exit
+    if (THETA == 3 && !a.state->near) return;
 
     const int mu = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -429,7 +489,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     }
 
     Philox rng;
-    if (A == GF_SA || A == GF_SR || A == GF_RANDOM) {
+    constexpr int AB = A % kDpUpdate;  // the update rule without its precision
+    if (AB == GF_SA || AB == GF_SR || AB == GF_RANDOM) {
         // key word 0 = global thread id of the reference's (mu=0, down=0) thread
         const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
         philox_init(rng, tid, a.seed, a.counter);
@@ -461,6 +522,16 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs<R> a)
     };
 
     if (THETA == 1) theta_write();
+    if constexpr (THETA == 3) {
+        theta_write();
+        __syncthreads();
+        theta_read();
+        if (threadIdx.x == 0) {
+            a.partial[2 * blockIdx.x] = th;
+            a.partial[2 * blockIdx.x + 1] = gf;
+        }
+        return;
+    }
 
     auto subgroup = [&](auto I_, auto J_, int buf) {
         constexpr int I = decltype(I_)::value, J = decltype(J_)::value;

@@ -34,6 +34,14 @@ def oracle32():
 
 
 @pytest.fixture(scope="session")
+def oracle32dp():
+    """Float links with the OR, microcanonical and heatbath updates in double
+    (the reference's DPUPDATES build)."""
+    from oracle_lib import Oracle
+    return Oracle("f32dp")
+
+
+@pytest.fixture(scope="session")
 def ref():
     """The reference-header harness (oracle/_ref); skipped where it is not built."""
     from oracle_lib import RefHarness

@@ -38,6 +38,7 @@ class Oracle:
         path = ORACLE_DIR / f"liboracle_{kind}.so"
         _ensure_built(path)
         self.lib = C.CDLL(str(path))
+        self.kind = kind
         self.real = C.c_double if kind == "f64" else C.c_float
         self.np_real = np.float64 if kind == "f64" else np.float32
         L = self.lib

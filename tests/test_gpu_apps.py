@@ -83,6 +83,18 @@ def test_landau_or_only_f32(gf, gpu, oracle32):
     assert got[0]["theta"] < 1e-6
 
 
+def test_landau_or_only_f32_dp_updates(gf, gpu, oracle32dp):
+    """The single-precision build with DPUPDATES=true (-DUSE_DP_ORUPDATE
+    -DUSE_DP_MICROUPDATE -DUSE_DP_SAUPDATE, apps/Makefile:46-48): the drop-in
+    headers select the double-precision update rules, and the OR loop stops
+    on the oracle's sweep with its functional."""
+    o = Options(randomtrafo=0, sasteps=0, precision=1e-6, ormaxiter=3000, checkprecision=10)
+    got, _ = run_app("gaugefix_x8t8_f32dp", "landau", o)
+    want = run_4d(oracle32dp, "landau", DIMS, config(gf, np.float32), o)
+    compare(got, want, 1e-6)
+    assert got[0]["theta"] < 1e-6
+
+
 def test_landau_sa_copies_f64(gf, gpu, oracle):
     """Random transformation, 30 SA temperatures with 3 microcanonical steps
     each, then OR to 1e-12, for two gauge copies."""

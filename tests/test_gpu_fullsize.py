@@ -1,6 +1,6 @@
 """The benchmark's own lattices through size-independent properties: 32^4
 Landau (sigma 0.25, seed 1, Haar seed 2), the 32^3 x 64 Coulomb lattice and
-the gauge copies' 16^4 and 20^4 lattices with the full annealing schedule.
+the gauge copies' 16^4 to 32^4 lattices with the full annealing schedule.
 The CPU oracle would need minutes for one run at these sizes, so the checks
 compare the GPU paths with each other (the oracle only measures the results).
 
@@ -96,22 +96,24 @@ def test_coulomb_32x32x32x64_every_slice(gf, gpu, oracle):
         assert gm == pytest.approx(g64, rel=1e-10)
 
 
-@pytest.mark.parametrize("L", [16, 20])
-def test_gauge_copies_full_schedule(gf, gpu, L):
+@pytest.mark.parametrize("L,copies", [(16, 4), (20, 4), (24, 2), (32, 1)])
+def test_gauge_copies_full_schedule(gf, gpu, L, copies):
     """The gauge-copies workload's L^4 copies (sigma 0.3, seed 1, Haar seeds
     10.. one per copy, seed 1 with counter ranges 100000 apart) through the
     batch calls with the full schedule: annealing T 1.45 -> 0.01 in 3000 steps
     with 3 microcanonical steps each (FP32 gauge annealing on 24-bit links),
     then mixed OR to theta < 1e-12.  Every copy converges; the copies end on
     one functional to 1e-12 (this smooth configuration has one maximum); each
-    result is a gauge transformation of its input (plaquette to 1e-12)."""
+    result is a gauge transformation of its input (plaquette to 1e-12).  The
+    workload's four copies of 16^4 and 20^4, two of 24^4 and one of 32^4 (the
+    sizes at which a copy fills the GPU alone run one after another)."""
     dims = (L, L, L, L)
     V = L ** 4
-    cfgs = [gf.make_config(dims, 0.3, 1, 10 + c) for c in range(4)]
+    cfgs = [gf.make_config(dims, 0.3, 1, 10 + c) for c in range(copies)]
     bufs = [gf.DeviceBuffer.from_numpy(u) for u in cfgs]
-    lats = [gf.Lattice(dims)] * 4
-    sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, 3000, 3, 100, seeds=[1] * 4,
-                         counters=[c * 100000 for c in range(4)], mode=2)
+    lats = [gf.Lattice(dims)] * copies
+    sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, 3000, 3, 100, seeds=[1] * copies,
+                         counters=[c * 100000 for c in range(copies)], mode=2)
     assert all(t == 3000 for _, t in sa)
     res = gf.batch_run_mixed(lats, bufs, gf.LANDAU, 1.7, 1e-12, 1e-12, 20000, check32=5)
     print(L, [r.gff for r in res], flush=True)

@@ -100,6 +100,69 @@ def test_sweep_bit_exact(gf, gpu, oracle, gauge, algo, dims):
     assert not np.array_equal(rows01(Ug, V), rows01(U0, V))
 
 
+@pytest.mark.parametrize("dp", [False, True])
+@pytest.mark.parametrize("gauge", [LANDAU, MAG, U1XU1])
+@pytest.mark.parametrize("algo,param", [(OR, 1.7), (MICRO, 0.0), (SA, 0.4)])
+def test_fp32_sweeps(gf, gpu, oracle32, oracle32dp, gauge, algo, param, dp):
+    """FP32 links (the reference's default build), and FP32 links with the
+    update rule in double (GF_UPDATE_DP, its DPUPDATES build): three half-sweep
+    pairs against the oracle's float and DPUPDATES builds, bit for bit for OR
+    and microcanonical steps; the heatbath's log / cospi / sinpi are the
+    device's and the host's, equal to float rounding."""
+    oracle = oracle32dp if dp else oracle32
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims, gf.F32)
+    U0 = _config(gf, dims).astype(np.float32)
+    Uo = U0.copy()
+    d = gf.DeviceBuffer.from_numpy(U0)
+    K = getattr(gf, KERNELS[gauge])(lat, dp_updates=dp)
+    a, b = V // 2 // 32, 256
+    counter = 11
+    for _ in range(3):
+        for parity in (0, 1):
+            if algo == OR:
+                K.orStep(a, b, d, None, parity, param)
+            elif algo == MICRO:
+                K.microStep(a, b, d, None, parity)
+            else:
+                K.saStep(a, b, d, None, parity, param, 7, counter)
+            oracle.sweep(gauge, algo, dims, Uo, parity, param, 7, counter)
+            counter += 1
+    Ug = d.download(np.float32, U0.size)
+    if algo == SA:
+        # U1xU1 keeps (q0, q3) of the heatbath's element and normalises them,
+        # which magnifies a last-ulp difference of the float logarithm
+        tol = 1e-4 if gauge == U1XU1 and not dp else 1e-5
+        assert np.abs(rows01(Ug, V) - rows01(Uo, V)).max() < tol
+    else:
+        assert np.array_equal(rows01(Ug, V), rows01(Uo, V)), np.abs(rows01(Ug, V) - rows01(Uo, V)).max()
+    assert not np.array_equal(rows01(Ug, V), rows01(U0, V))
+
+
+def test_fp32_dp_update_differs_from_fp32_update(gf, gpu, oracle32):
+    """The flag changes the arithmetic: FP32 OR sweeps with the update in
+    double leave the links a few float ulps away from the float-update sweeps
+    (which the float oracle reproduces bit for bit)."""
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims, gf.F32)
+    U0 = _config(gf, dims).astype(np.float32)
+    out = {}
+    for dp in (False, True):
+        d = gf.DeviceBuffer.from_numpy(U0)
+        K = gf.LandauKernelsSU3(lat, dp_updates=dp)
+        for parity in (0, 1, 0, 1):
+            K.orStep(V // 2 // 32, 256, d, None, parity, 1.7)
+        out[dp] = rows01(d.download(np.float32, U0.size), V)
+    Uo = U0.copy()
+    for parity in (0, 1, 0, 1):
+        oracle32.sweep(LANDAU, OR, dims, Uo, parity, 1.7)
+    assert np.array_equal(out[False], rows01(Uo, V))
+    diff = np.abs(out[True] - out[False]).max()
+    assert 0 < diff < 1e-5, diff
+
+
 def test_sr_bit_exact(gf, gpu, oracle):
     dims = (4, 6, 4, 8)
     U0, Uo, Ug, V = _run_parity(gf, oracle, MAG, SR, dims, nsweeps=3, param=0.5)

@@ -750,3 +750,72 @@ def test_mag_theta_vanishes_at_converged_maximum(oracle, gf):
             for eps in (1e-3, -1e-3):
                 R = _su2_embed(i, j, k, eps)
                 assert (_mag_local(R @ up, dn @ np.conj(R.T)) - base).max() < 1e-12
+
+
+# ------------------------------------------------ double-precision updates
+def _exact_square_quats(n, seed):
+    """Float quaternions whose components are multiples of 2^-6 in (-2, 2):
+    their squares and the sums of three squares are exact in float, so the
+    reference's float-typed squares (shA is Real) lose nothing and a
+    double-precision update is the float rounding of the exact result."""
+    rng = np.random.default_rng(seed)
+    q = rng.integers(-127, 128, size=(n, 4)).astype(np.float64) / 64.0
+    q[np.abs(q).sum(axis=1) == 0] = 1.0
+    return q
+
+
+def test_dp_updates_are_rounded_double_results(oracle32, oracle32dp):
+    """DPUPDATES (OrUpdate.hxx:54-64, MicroUpdate.hxx:45-54): on float links
+    the update is computed in double and rounded once, so on inputs with exact
+    squares OR with omega = 1 gives float(a/|a|) and the microcanonical update
+    float((a/|a|)^2) exactly (closed forms, computed here in float64 without
+    the update's formula); the float build misses these by an ulp or more on
+    some inputs, so the test tells the two precisions apart."""
+    dp_or = dp_mi = sp_or = sp_mi = 0
+    for a in _exact_square_quats(400, 21):
+        ah = a / np.linalg.norm(a)
+        want_or = ah.astype(np.float32)
+        want_mi = _hamilton(ah, ah).astype(np.float32)
+        dp_or += not np.array_equal(oracle32dp.update(OR, a, 1.0), want_or)
+        dp_mi += not np.array_equal(oracle32dp.update(MICRO, a), want_mi)
+        sp_or += not np.array_equal(oracle32.update(OR, a, 1.0), want_or)
+        sp_mi += not np.array_equal(oracle32.update(MICRO, a), want_mi)
+    assert dp_or == 0 and dp_mi == 0, (dp_or, dp_mi)
+    assert sp_or > 20 and sp_mi > 20, (sp_or, sp_mi)
+
+
+def test_dp_or_update_follows_reference_typing(oracle32dp):
+    """OrUpdate.hxx:54-64 on general float input: the squares are float sums
+    (the reference assigns a float expression to a double), b, c and the
+    products double, each result rounded to float; omega = 1.7 keeps the
+    vector part parallel to a's and the result a unit quaternion to float
+    precision."""
+    rng = np.random.default_rng(5)
+    for a in rng.normal(size=(300, 4)).astype(np.float32):
+        ai = np.float64(a[1] * a[1] + a[2] * a[2] + a[3] * a[3])
+        a0 = np.float64(a[0] * a[0])
+        w = np.float64(np.float32(1.7))
+        b = (w * a0 + ai) / (a0 + ai)
+        c = 1.0 / np.sqrt(a0 + b * b * ai)
+        want = np.array([np.float64(a[0]) * c] + [np.float64(x) * (b * c) for x in a[1:]]).astype(np.float32)
+        g = oracle32dp.update(OR, a, 1.7)
+        assert np.array_equal(g, want), (a, g, want)
+        assert abs(np.linalg.norm(g.astype(np.float64)) - 1.0) < 3e-7
+        np.testing.assert_allclose(np.cross(g[1:].astype(np.float64), a[1:].astype(np.float64)), 0.0, atol=1e-6)
+
+
+@pytest.mark.parametrize("norm,temperature", [(2.0, 1.0), (3.0, 0.375)])
+def test_dp_sa_update_is_heatbath(oracle32dp, norm, temperature):
+    """SaUpdate.hxx:54-98 (USE_DP_SAUPDATE): the heatbath in double on float
+    uniforms has the Kennedy-Pendleton distribution, and its results are unit
+    quaternions to float precision."""
+    n = 8000
+    a = np.array([0.4, -0.7, 0.5, 0.3])
+    a = (a / np.linalg.norm(a) * norm).astype(np.float32)
+    ah = a.astype(np.float64) / np.linalg.norm(a.astype(np.float64))
+    g = _samples(oracle32dp, SA, a, temperature, n).astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(g, axis=1), 1.0, atol=3e-7)
+    x = g @ ah
+    k = np.linalg.norm(a.astype(np.float64)) / temperature
+    assert _ks_pvalue(x, _kp_cdf(k)) > 1e-3
+    assert _ks_pvalue(x, _kp_cdf(1.5 * k)) < 1e-6

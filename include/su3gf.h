@@ -256,6 +256,22 @@ gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float 
 gf_status gf_run_fast(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
                       int reproject, gf_run_result *out, void *stream);
 
+/* gf_run_fast without any allocation, for callers that cannot allocate
+ * device or host memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * caller passes the links and the result as for gf_run_fast; the run's device
+ * state and its per-block theta/gff partials live in a fixed array of the
+ * library (4 MiB + 256 bytes on each GPU it runs on, kept until the process
+ * ends), so lattices of at most 131072 64-site blocks per half-sweep
+ * (16,777,216 sites, 64^4) are accepted, larger ones rejected with
+ * GF_ERR_BAD_SHAPE.  It sweeps the caller's links in place (no tiled copy)
+ * and returns the same sweeps, theta, gff and links as gf_run_fast.
+ * out->ms_total is host wall time (no events are created).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams
+ * or threads) race and must not be made. */
+gf_status gf_run_fast_noalloc(const gf_lattice *lat, void *U, int gauge, float omega, float precision, int maxiter,
+                              int reproject, gf_run_result *out, void *stream);
+
 /* Average duration of one sweep kernel as the drivers launch it, from HIP
  * events around nsweeps back-to-back OR sweeps on `stream` (after
  * max(2, nsweeps / 2) warm-up sweeps).  variant 0: the link sweep in the precision of lat with

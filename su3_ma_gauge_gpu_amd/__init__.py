@@ -62,7 +62,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_fast", "run_mixed", "run_sa", "batch_run_mixed", "batch_run_sa", "device_info",
+    "make_config", "run_or", "run_fast", "run_fast_noalloc", "run_mixed", "run_sa", "batch_run_mixed", "batch_run_sa", "device_info",
     "RunResult", "LinkFile",
 ]
 
@@ -417,6 +417,17 @@ def run_fast(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, preci
     r = _lib.gf_run_result()
     _lib.call("gf_run_fast", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega), C.c_float(precision),
               int(maxiter), int(reproject), C.byref(r), _ptr(stream))
+    return _result(r)
+
+
+def run_fast_noalloc(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, precision: float = 1e-12,
+                     maxiter: int = 20000, reproject: int = 100, stream=None) -> RunResult:
+    """run_fast without any allocation (its state lives in a fixed device
+    array of the library: at most 64^4 sites; calls overlapping on one GPU
+    race); see gf_run_fast_noalloc in include/su3gf.h."""
+    r = _lib.gf_run_result()
+    _lib.call("gf_run_fast_noalloc", C.byref(lattice.struct), _ptr(U), gauge, C.c_float(omega),
+              C.c_float(precision), int(maxiter), int(reproject), C.byref(r), _ptr(stream))
     return _result(r)
 
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "gf_quality_per_site": (_i, [_pl, _i, _vp, _vp, _vp, _vp]),
     "gf_run_or": (_i, [_pl, _vp, _i, _f, _f, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_run_fast": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
+    "gf_run_fast_noalloc": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -706,10 +707,15 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
 // ------------------------------------------------- fused-check drivers
 namespace {
 
+// bytes of a fused run's device scratch for `blocks` blocks per parity: the
+// DevState (in 256 bytes), then (theta, gff) of every block of each parity
+constexpr size_t fused_scratch_bytes(int64_t blocks) { return 256 + sizeof(double) * 4 * (size_t)blocks; }
+
 struct FusedCtx {
     Scratch dev;                // DevState, then 2 * nblocks doubles for each parity
     DevState *st = nullptr;     // device
-    DevState *host = nullptr;   // pinned
+    DevState *host = nullptr;   // pinned, or own_host
+    DevState own_host;          // the allocation-free driver's (pageable) copy
     double *partial = nullptr;
     double *even() { return partial; }
     double *odd() { return partial + 2 * nblocks; }
@@ -717,11 +723,20 @@ struct FusedCtx {
     gf_status init(int64_t blocks, hipStream_t s)  // partial sums for up to `blocks` blocks per parity
     {
         nblocks = blocks;
-        HIP_TRY(dev.alloc(256 + sizeof(double) * 4 * nblocks, s));  // state, even, odd
+        HIP_TRY(dev.alloc(fused_scratch_bytes(nblocks), s));  // state, even, odd
         st = dev.as<DevState>();
         partial = (double *)((char *)dev.p + 256);
         HIP_TRY(pinned_state(&host));
         return GF_OK;
+    }
+    // the same layout in caller-provided device memory (fused_scratch_bytes(blocks)
+    // bytes); the host copy of the state is this object's own
+    void init_at(void *mem, int64_t blocks)
+    {
+        nblocks = blocks;
+        st = (DevState *)mem;
+        partial = (double *)((char *)mem + 256);
+        host = &own_host;
     }
     gf_status read(hipStream_t s)
     {
@@ -1207,3 +1222,85 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free OR driver
+// gf_run_fast for callers that cannot allocate: the run's device state and
+// block partials live in `scratch`, a file-scope device array that each GPU
+// holds once for the life of the process.  Host code never passes the
+// array's name to a kernel (there it is a host-side stand-in): the device
+// address of each GPU's copy comes from hipGetSymbolAddress, taken once per
+// device on the first call with that device current.
+namespace {
+constexpr int64_t kNoAllocBlocks = 131072;  // 64-site blocks per half-sweep: up to 16,777,216 sites (64^4)
+constexpr size_t kNoAllocScratchBytes = fused_scratch_bytes(kNoAllocBlocks);
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+gf_status noalloc_scratch(void **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GF_ERR_BAD_ARG, "device %d beyond the %d the library tracks", dev,
+                                                    kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GF_ERR_HIP, "hipGetSymbolAddress returned a null address for the scratch array");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return GF_OK;
+}
+}  // namespace
+
+extern "C" gf_status gf_run_fast_noalloc(const gf_lattice *lat, void *U, int gauge, float omega, float precision,
+                                         int maxiter, int reproject, gf_run_result *out, void *stream)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
+    if (gauge == GF_COULOMB && lat->dims[0] == 1)
+        return fail(GF_ERR_BAD_ARG, "gf_run_fast_noalloc fixes every timeslice of a 4D lattice at once");
+    if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
+    if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
+        return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
+    const int64_t nblocks = sweep_grid(per_slice(make_geom(lat->dims, V), gauge));
+    if (nblocks > kNoAllocBlocks)
+        return fail(GF_ERR_BAD_SHAPE,
+                    "gf_run_fast_noalloc: %lld blocks of 64 sites per half-sweep exceed its fixed scratch of %lld "
+                    "(%zu bytes; at most 16777216 sites, 64^4); use gf_run_fast",
+                    (long long)nblocks, (long long)kNoAllocBlocks, kNoAllocScratchBytes);
+    memset(out, 0, sizeof *out);
+    hipStream_t s = S(stream);
+    void *mem = nullptr;
+    GF_TRY(noalloc_scratch(&mem));
+    FusedCtx fc;
+    fc.init_at(mem, nblocks);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto after = [&](int it) -> gf_status {
+        if (reproject > 0 && it % reproject == 0) return project_unless_done(lat, U, fc.st, s);
+        return GF_OK;
+    };
+    int done = 0;
+    gf_status rc = lat->dtype == GF_F64
+                       ? fused_phase<double>(lat, (double *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, true,
+                                             false, fc, s, after, &done, out)
+                       : fused_phase<float>(lat, (float *)U, nullptr, gauge, omega, precision, maxiter, 1, 0, 0, true,
+                                            false, fc, s, after, &done, out);
+    GF_TRY(rc);
+    out->sweeps = done;
+    // third rows (LandauGaugeFixingSU3_4D.cu:271), projected in place as
+    // gf_run_fast projects its copy on the way back
+    if (lat->dtype == GF_F64) HIP_TRY(launch_convert(GF_F64, lat->layout, U, GF_F64, lat->layout, U, V, 1, s));
+    else GF_TRY(gf_project(lat, U, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!out->converged) GF_TRY(final_quality(lat, U, gauge, fc, out, s));
+    return GF_OK;
+}

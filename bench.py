@@ -16,7 +16,12 @@ copy is inside the timed region).
 Printed: one JSON line with the metric, the roofline figure of the dominant
 kernel (algorithmic bytes per launch / launch time measured here with HIP
 events on the launch stream) and the CPU baseline (the FP64 oracle on this
-host's cores, a bounded number of 32^4 sweeps).
+host's cores, a bounded number of 32^4 sweeps).  Under "workloads" the same
+line carries one run of each other workload, outside the timed steps:
+Coulomb 32^3 x 64 (every slice at once), the gauge copies (4 each of 16^4 to
+32^4, annealing then OR), MAG to theta < 1e-12 at 24^4 then U1xU1 (with MAG's
+kernels timed at 48^4) and Landau at 48^4, where the lattice streams from
+HBM instead of the Infinity Cache (--no-extras leaves them out).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--dims 32 32 32 32]
 """
@@ -61,12 +66,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="leave out the other workloads (Coulomb, gauge copies, MAG, Landau 48^4) of the default line")
     ap.add_argument("--mode", type=int, default=3, help="0 links+g, 1 g (FP32 arithmetic), 2 g (FP64 arithmetic), 3 g on 24-bit fixed-point links")
     ap.add_argument("--copies", type=int, default=4)
     ap.add_argument("--sa-steps", type=int, default=3000)
     ap.add_argument("--sa-mode", type=int, default=2,
                     help="0 FP64 link sweeps, 1 FP32 gauge-transformation sweeps, 2 the same on 24-bit links")
     ap.add_argument("--mag-precision", type=float, default=1e-12)
+    ap.add_argument("--maxiter-mag", type=int, default=600000)
+    ap.add_argument("--u1xu1-sweeps", type=int, default=1000)
     ap.add_argument("--workload", default="landau",
                     choices=["landau", "coulomb", "gribov", "mag"],
                     help="landau: the 32^4 headline run (default); coulomb: 32^3x64 all slices at once; "
@@ -202,6 +211,9 @@ def coulomb_line(args) -> dict:
         return gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.switch, args.precision, args.maxiter,
                             mode=args.mode, check32=args.check32, stream=stream)
 
+    # the library picks its wavefront slab for a lattice on its first run
+    reset()
+    gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.switch, args.precision, 2, mode=args.mode, stream=stream)
     for _ in range(args.warmup):
         mixed()
     rs = [mixed() for _ in range(args.steps)]
@@ -228,11 +240,50 @@ def coulomb_line(args) -> dict:
 
 
 def workload_mag(args):
-    """MAG of a 48^4 lattice (sigma 0.35, seed 1, Haar seed 2): mixed (FP32
-    phase in --mode, switch at theta < 1e-6) against all-FP64 to theta <
-    precision, then 1000 U1xU1 OR sweeps on the MAG result, in FP64 and in
-    mixed precision."""
-    dims = (48, 48, 48, 48)
+    """MAG of a 48^4 lattice (sigma 0.35, seed 1, Haar seed 2), the full
+    schedule: mixed (FP32 phase in --mode, switch at theta < 1e-6, then FP64)
+    against all-FP64 to theta < --mag-precision, then U1xU1 OR sweeps on the
+    MAG result; MAG's sweep kernels timed one by one.  At this size OR needs
+    a few hundred thousand sweeps: expect minutes per run."""
+    line = mag_line(args, 48, args.maxiter_mag)
+    line.update(mag_kernels(args, 48))
+    print(json.dumps(line), flush=True)
+
+
+def _run_chunked(lat, work, gauge, args, prec, maxiter, stream, what):
+    """gf_run_fast to theta < prec in chunks of 20000 sweeps (a progress line
+    after each): one run, as the sweeps do not depend on where a chunk ends
+    but for the reprojection at each chunk's first sweep.  Returns (sweeps,
+    ms, last result)."""
+    sweeps, ms, r = 0, 0.0, None
+    while sweeps < maxiter:
+        r = gf.run_fast(lat, work, gauge, args.omega, prec, min(20000, maxiter - sweeps), args.reproject,
+                        stream=stream)
+        sweeps += r.sweeps
+        ms += r.ms_total
+        print(f"bench: {what}: {sweeps} sweeps, theta {r.theta:.3e}, {ms / 1e3:.1f} s", file=sys.stderr, flush=True)
+        if r.converged:
+            break
+    return sweeps, ms, r
+
+
+def mag_quality(lat, U, stream):
+    """MAG gff and theta of a lattice (GaugeFixingStats with MAGKernelsSU3's
+    prefactors, the drop-in quality kernel)."""
+    st = gf.GaugeFixingStats(gf.MAGKernelsSU3, U, lat, stream=stream)
+    st.generateGaugeQuality()
+    g, t = st.getCurrentGff(), st.getCurrentA()
+    st.close()
+    return g, t
+
+
+def mag_line(args, L: int, maxiter: int) -> dict:
+    """MAG of an L^4 lattice (sigma 0.35, seed 1, Haar seed 2) to theta <
+    --mag-precision: mixed (the FP32 phase in --mode until theta < 1e-6, then
+    FP64 sweeps) against all-FP64; then --u1xu1-sweeps U1xU1 OR sweeps on
+    each MAG result, which leave MAG's functional and theta unchanged (a
+    diagonal gauge rotation changes neither |U_jj| nor |X_k|)."""
+    dims = (L, L, L, L)
     V = int(np.prod(dims))
     U0 = gf.make_config(dims, 0.35, 1, 2)
     lat = gf.Lattice(dims)
@@ -244,43 +295,64 @@ def workload_mag(args):
     def reset():
         _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
 
+    # mixed: the FP32 phase to 1e-6 (gf_run_mixed with precision 1e-6 ends on
+    # its first FP64 check), then FP64 sweeps to the precision
     reset()
-    m = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, prec, args.maxiter, mode=args.mode,
-                     refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
-    print(f"bench: mag mixed {m}", file=sys.stderr, flush=True)
-    # U1xU1 on the MAG result, 1000 sweeps in FP64 and in mixed precision
-    # (the FP32 phase takes all of them: U1xU1's theta stays far above any
-    # switch threshold) from the same start
-    mag_fixed = gf.DeviceBuffer(lat.nbytes)
-    _lib.call("gf_memcpy_async", C.c_void_p(mag_fixed.ptr), C.c_void_p(work.ptr), C.c_size_t(lat.nbytes), stream)
-    u = gf.run_fast(lat, work, gf.U1XU1, args.omega, 1e-30, 1000, args.reproject, stream=stream)
-    print(f"bench: u1xu1 {u}", file=sys.stderr, flush=True)
-    _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(mag_fixed.ptr), C.c_size_t(lat.nbytes), stream)
-    um = gf.run_mixed(lat, work, gf.U1XU1, args.omega, 1e-30, 1e-30, 1000, mode=args.mode, refresh=args.refresh,
-                      check32=args.check32, reproject=args.reproject, stream=stream)
-    print(f"bench: u1xu1 mixed {um}", file=sys.stderr, flush=True)
-    del mag_fixed
+    m32 = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, 1e-6, maxiter, mode=args.mode, refresh=args.refresh,
+                       check32=args.check32, reproject=args.reproject, stream=stream)
+    print(f"bench: mag {L}^4 mixed, FP32 phase: {m32}", file=sys.stderr, flush=True)
+    n64, ms64, m = _run_chunked(lat, work, gf.MAG, args, prec, maxiter - m32.sweeps, stream, f"mag {L}^4 mixed, FP64")
+    mixed = {"time_to_theta_s": (m32.ms_total + ms64) / 1e3, "sweeps": m32.sweeps + n64,
+             "sweeps_fp32": m32.sweeps_fp32, "time_fp32_s": m32.ms_fp32 / 1e3, "converged": m.converged,
+             "gff": m.gff, "theta": m.theta}
+    mixed_fixed = gf.DeviceBuffer(lat.nbytes)
+    _lib.call("gf_memcpy_async", C.c_void_p(mixed_fixed.ptr), C.c_void_p(work.ptr), C.c_size_t(lat.nbytes), stream)
     reset()
-    f = gf.run_fast(lat, work, gf.MAG, args.omega, prec, args.maxiter, args.reproject, stream=stream)
-    print(f"bench: mag fp64 {f}", file=sys.stderr, flush=True)
-    line = _base_line(f"MAG OR site-updates/s + time to theta<{prec:g} at 48^4: mixed vs FP64, then U1xU1",
-                      m.sweeps * V / (m.ms_total / 1e3), "site-updates/s", m.ms_total, 1, 0,
-                      {"workload": "mag-or-48x48x48x48-mixed+u1xu1", "dims": list(dims), "sigma": 0.35, "seed": 1,
-                       "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
-                       "mode": MODE_NAMES[args.mode]})
+    nf, msf, f = _run_chunked(lat, work, gf.MAG, args, prec, maxiter, stream, f"mag {L}^4 FP64")
+    fp64 = {"time_to_theta_s": msf / 1e3, "sweeps": nf, "converged": f.converged, "gff": f.gff, "theta": f.theta}
+    # U1xU1 on each MAG result
+    u1 = {}
+    for name, buf in (("fp64", work), ("mixed", mixed_fixed)):
+        g0, t0 = mag_quality(lat, buf, stream)
+        if name == "fp64":
+            u = gf.run_fast(lat, buf, gf.U1XU1, args.omega, 1e-30, args.u1xu1_sweeps, args.reproject, stream=stream)
+        else:
+            u = gf.run_mixed(lat, buf, gf.U1XU1, args.omega, 1e-30, 1e-30, args.u1xu1_sweeps, mode=args.mode,
+                             refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
+        g1, t1 = mag_quality(lat, buf, stream)
+        u1[name] = {"sweeps": u.sweeps, "time_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
+                    "mag_gff_rel_change": abs(g1 - g0) / g0, "mag_theta_before": t0, "mag_theta_after": t1}
+        print(f"bench: u1xu1 on the {name} MAG result ({L}^4): {u}", file=sys.stderr, flush=True)
+    del mixed_fixed
+    line = _base_line(f"MAG OR time to theta<{prec:g} at {L}^4: mixed vs FP64, then U1xU1",
+                      mixed["sweeps"] * V / mixed["time_to_theta_s"], "site-updates/s", mixed["time_to_theta_s"] * 1e3,
+                      1, 0,
+                      {"workload": f"mag-or-{L}x{L}x{L}x{L}-mixed+u1xu1", "dims": list(dims), "sigma": 0.35,
+                       "seed": 1, "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
+                       "mode": MODE_NAMES[args.mode], "u1xu1_sweeps": args.u1xu1_sweeps})
+    both = mixed["converged"] and fp64["converged"]
     line.update({
-        "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
-        "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,
-        "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
-        "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
-        "speedup_mixed_vs_fp64": f.ms_total / m.ms_total,
-        "u1xu1_1000_sweeps_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
-        "u1xu1_1000_sweeps_mixed_s": um.ms_total / 1e3, "u1xu1_gff_mixed": um.gff,
-        "u1xu1_gff_rel_diff_mixed_vs_fp64": abs(um.gff - u.gff) / abs(u.gff),
+        "mixed": mixed, "fp64": fp64,
+        "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
+        # a time-to-theta ratio only when both runs reached the precision
+        "speedup_mixed_vs_fp64": fp64["time_to_theta_s"] / mixed["time_to_theta_s"] if both else None,
+        "u1xu1": u1,
+        "u1xu1_gff_rel_diff_mixed_vs_fp64": abs(u1["mixed"]["u1xu1_gff"] - u1["fp64"]["u1xu1_gff"])
+        / abs(u1["fp64"]["u1xu1_gff"]),
     })
-    # MAG's sweep kernels, one parity launch each (V/2 site-updates), and their
-    # algorithmic bytes as a fraction of 8 TB/s (bytes per site-update as in
-    # the Landau kernel table: the update rule does not change what is read)
+    return line
+
+
+def mag_kernels(args, L: int) -> dict:
+    """MAG's sweep kernels at L^4, one parity launch each (V/2 site-updates),
+    and their algorithmic bytes as a fraction of 8 TB/s (bytes per
+    site-update as in the Landau kernel table: the update rule does not change
+    what is read)."""
+    dims = (L, L, L, L)
+    V = int(np.prod(dims))
+    lat = gf.Lattice(dims)
+    src = gf.DeviceBuffer.from_numpy(gf.make_config(dims, 0.35, 1, 2))
+    stream = _stream()
     lat32 = gf.Lattice(dims, gf.F32, gf.R12)
     U32 = gf.DeviceBuffer(lat32.nbytes)
     kms, frac = {}, {}
@@ -291,19 +363,20 @@ def workload_mag(args):
                       C.c_void_p(U32.ptr), 0, stream)
             ms = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var, gf.MAG)
         else:
-            reset()
-            ms = kernel_time_ms(lat, work, stream, args.kernel_sweeps, 0, gf.MAG)
+            ms = kernel_time_ms(lat, src, stream, args.kernel_sweeps, 0, gf.MAG)
         kms[name] = ms
         frac[name] = (V / 2) * bsu / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-    line.update({"kernel_ms": kms, "kernel_hbm_frac": frac})
-    print(json.dumps(line), flush=True)
+    return {f"kernel_ms_{L}": kms, f"kernel_hbm_frac_{L}": frac}
 
 
 def workload_gribov(args):
-    print(json.dumps(gribov_line(args)), flush=True)
+    streams = [_stream() for _ in range(args.copies)]
+    line = gribov_line(args, streams)
+    line["with_sa_graph"] = gribov_line(args, streams, graph=True)
+    print(json.dumps(line), flush=True)
 
 
-def gribov_line(args, streams=None) -> dict:
+def gribov_line(args, streams=None, graph=False) -> dict:
     """Gauge copies: for each of 16^4, 20^4, 24^4, 32^4 (sigma 0.3, seed 1),
     four copies with Haar seeds 10.. (one per copy); each copy is annealed
     (T 1.45 -> 0.01 in args.sa_steps steps, 3 microcanonical steps per
@@ -314,7 +387,9 @@ def gribov_line(args, streams=None) -> dict:
     host thread of its own; each copy has its own RNG counter range.  The
     streams are created here, before the first device allocation of the
     workload (streams created after it have run the concurrent copies up to
-    35% slower with the runtime's 4 hardware queues per process)."""
+    35% slower with the runtime's 4 hardware queues per process).  graph:
+    each copy's annealing schedule is captured into one HIP graph and
+    replayed (GF_SA_GRAPH) instead of launched kernel by kernel."""
     if streams is None:
         streams = [_stream() for _ in range(args.copies)]
     sizes = [16, 20, 24, 32]
@@ -331,7 +406,7 @@ def gribov_line(args, streams=None) -> dict:
         t0 = time.perf_counter()  # the inputs were generated and uploaded before t0
         sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject,
                              seeds=[1] * args.copies, counters=[(h - 10) * 100000 for h in hs], mode=args.sa_mode,
-                             max_concurrent=args.copies, streams=streams)
+                             max_concurrent=args.copies, streams=streams, graph=graph)
         t1 = time.perf_counter()
         res = gf.batch_run_mixed(lats, bufs, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
                                  mode=args.mode, check32=args.check32, max_concurrent=args.copies,
@@ -348,10 +423,99 @@ def gribov_line(args, streams=None) -> dict:
     line = _base_line("Gribov copies: site-updates/s over SA + OR of 4 copies each of 16^4..32^4",
                       total_su / total_s, "site-updates/s", total_s * 1e3, 1, 0,
                       {"workload": "gribov-copies-16-20-24-32-sa-or", "sizes": sizes, "copies": args.copies,
+                       "sa_graph": graph,
                        "sigma": 0.3, "seed": 1, "haar_seeds": [10, haar - 1], "sa": [1.45, 0.01, args.sa_steps, 3],
                        "precision": args.precision, "mode": MODE_NAMES[args.mode]})
     line.update({"total_s": total_s, "per_size": per_size})
     return line
+
+
+def landau48_line(args) -> dict:
+    """Landau gauge on a 48^4 lattice (sigma --sigma, seed 1, Haar seed 2):
+    the working set (3.06 GB of FP64 links; 0.33 GB of 24-bit links and g in
+    the FP32 phase) is far beyond the 256 MB Infinity Cache, so the kernels
+    stream from HBM itself.  Mixed against FP64 to theta < --precision, and
+    the dominant kernels' launch times with their fraction of 8 TB/s."""
+    dims = (48, 48, 48, 48)
+    V = int(np.prod(dims))
+    U0 = gf.make_config(dims, args.sigma, 1, 2)
+    lat64 = gf.Lattice(dims)
+    lat32 = gf.Lattice(dims, gf.F32, gf.R12)
+    src = gf.DeviceBuffer.from_numpy(U0)
+    work = gf.DeviceBuffer(lat64.nbytes)
+    stream = _stream()
+
+    def reset():
+        _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat64.nbytes), stream)
+
+    # a two-sweep run first: the library picks its wavefront slab for this
+    # lattice on its first run (a few timed sweeps), outside the timed runs
+    reset()
+    gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, 2, mode=args.mode, stream=stream)
+    reset()
+    m = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter, mode=args.mode,
+                     refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
+    print(f"bench: landau 48^4 mixed {m}", file=sys.stderr, flush=True)
+    slab = C.c_int()
+    _lib.call("gf_wavefront_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(slab))
+    # the same run with every sweep as two kernels (no wavefront)
+    prev = C.c_int()
+    _lib.call("gf_set_wavefront", 0, C.byref(prev))
+    reset()
+    m2 = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter, mode=args.mode,
+                      refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
+    _lib.call("gf_set_wavefront", prev.value, None)
+    print(f"bench: landau 48^4 mixed, two-kernel sweeps {m2}", file=sys.stderr, flush=True)
+    reset()
+    f = gf.run_fast(lat64, work, gf.LANDAU, args.omega, args.precision, args.maxiter, args.reproject, stream=stream)
+    print(f"bench: landau 48^4 fp64 {f}", file=sys.stderr, flush=True)
+    U32 = gf.DeviceBuffer(lat32.nbytes)
+    _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(U32.ptr),
+              0, stream)
+    kms = {"gauge_sweep_q24": kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, 4),
+           "gauge_sweep_q24_wavefront": kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, 6),
+           "sweep_f64_tiled12": kernel_time_ms(lat64, src, stream, args.kernel_sweeps, 0),
+           "dropin_quality_f64_pattern18": kernel_time_ms(lat64, src, stream, args.kernel_sweeps, 5)}
+    # per half-sweep (V/2 site-updates); the wavefront's figure is half a
+    # whole sweep's time, so its fraction counts the algorithmic bytes of the
+    # half-sweeps it replaces
+    kbytes = {"gauge_sweep_q24": (V // 2) * 432, "gauge_sweep_q24_wavefront": (V // 2) * 432,
+              "sweep_f64_tiled12": (V // 2) * 16 * 12 * 8, "dropin_quality_f64_pattern18": V * 4 * 12 * 8}
+    both = m.converged == 1 and f.converged == 1
+    return {
+        "config": {"workload": "landau-or-48x48x48x48-mixed", "dims": list(dims), "sigma": args.sigma,
+                   "precision": args.precision, "switch_precision": args.switch, "mode": MODE_NAMES[args.mode]},
+        "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
+        "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,
+        "wavefront_slab": slab.value, "time_to_theta_mixed_two_kernel_sweeps_s": m2.ms_total / 1e3,
+        "mixed_identical_with_and_without_wavefront": (m.sweeps, m.gff, m.theta) == (m2.sweeps, m2.gff, m2.theta),
+        "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
+        "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
+        "speedup_mixed_vs_fp64": f.ms_total / m.ms_total if both else None,
+        "site_updates_per_s_mixed": m.sweeps * V / (m.ms_total / 1e3),
+        "kernel_ms": kms,
+        "kernel_hbm_frac": {k: kbytes[k] / (kms[k] / 1e3) / 1e9 / HBM_PEAK_GBS for k in kms},
+    }
+
+
+def extra_workloads(args, streams) -> dict:
+    """The other workloads, one run each, for the default line: Coulomb
+    32^3 x 64, the gauge copies (full schedule), MAG to theta < 1e-12 at 24^4
+    (the 48^4 MAG run takes minutes: --workload mag) with MAG's kernels timed
+    at 48^4, and Landau at 48^4 (HBM-resident)."""
+    one = argparse.Namespace(**{**vars(args), "steps": 1, "warmup": 0})
+    out = {}
+    for name, fn in (("coulomb_32x32x32x64", lambda: coulomb_line(one)),
+                     ("gauge_copies", lambda: gribov_line(one, streams)),
+                     ("gauge_copies_sa_graph", lambda: gribov_line(one, streams, graph=True)),
+                     ("mag_24x24x24x24", lambda: {**mag_line(one, 24, 200000), **mag_kernels(one, 48)}),
+                     ("landau_48x48x48x48", lambda: landau48_line(one))):
+        t0 = time.perf_counter()
+        print(f"bench: workload {name}", file=sys.stderr, flush=True)
+        out[name] = fn()
+        out[name]["wall_s"] = time.perf_counter() - t0
+        _lib.call("gf_synchronize")
+    return out
 
 
 def main():
@@ -363,6 +527,9 @@ def main():
     if args.workload != "landau":
         {"coulomb": workload_coulomb, "mag": workload_mag, "gribov": workload_gribov}[args.workload](args)
         return
+    # the gauge copies' streams, created before the first device allocation
+    # (streams created after it have run the concurrent copies up to 35% slower)
+    copy_streams = None if args.no_extras else [_stream() for _ in range(args.copies)]
     dims = tuple(args.dims)
     V = int(np.prod(dims))
     print(f"bench: generating {dims} config", file=sys.stderr, flush=True)
@@ -555,6 +722,9 @@ def main():
         "wall_s": wall,
     }
     _lib.call("gf_stream_destroy", stream)
+    if not args.no_extras:
+        del src, work, U32
+        line["workloads"] = extra_workloads(args, copy_streams)
     print(json.dumps(line), flush=True)
 
 

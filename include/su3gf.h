@@ -220,7 +220,13 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
  * arithmetic as in the reference's single-precision build), then applied to U
  * in FP64 and U reprojected; returns when done.  mode 2: as mode 1 with the
  * copy of U in 24-bit fixed point (see GF_MIXED_GAUGE24).  Modes 1 and 2
- * take at most 44,739,200 sites, as the gauge modes of gf_run_mixed. */
+ * take at most 44,739,200 sites, as the gauge modes of gf_run_mixed.
+ * mode | GF_SA_GRAPH: the whole schedule (every launch, with its temperature
+ * and RNG counter) is captured into one HIP graph on `stream` (or a stream of
+ * the call's own when `stream` is null) and replayed once, so the host
+ * submits one graph instead of (2 + 2 micro) launches per heatbath step;
+ * bit-identical results.  Returns when done.  No reference counterpart. */
+#define GF_SA_GRAPH 0x10
 gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
                     int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
                     void *stream);
@@ -281,11 +287,32 @@ gf_status gf_run_fast_noalloc(const gf_lattice *lat, void *U, int gauge, float o
  * gauge-transformation sweep with FP32 / FP64 arithmetic (U is then the
  * read-only link copy), 4 the same as 2 on the links packed into 24-bit
  * fixed point; 5 the drop-in quality kernel (gf_stats_quality's per-site
- * pass, nsweeps launches, any dtype and layout; the time is per launch).
+ * pass, nsweeps launches, any dtype and layout; the time is per launch);
+ * 6 as 4, each sweep run as the mixed run's FP32 phase runs a sweep without
+ * a check (a wavefront over timeslices where gf_set_wavefront's choice is
+ * one), the time per half of a sweep.
  * U is modified by variants 0 and 1.  Any gauge but Coulomb on a single
  * timeslice. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
                          float *ms_per_kernel, void *stream);
+
+/* The gauge-transformation sweep (the FP32 phase of gf_run_mixed's gauge
+ * modes, gf_run_sa modes 1 and 2) runs each sweep without a check as a
+ * wavefront over timeslices when that is faster: launch k updates the even
+ * sites of a slab of timeslices and the odd sites of the slab before it,
+ * shifted back by one timeslice, so a timeslice's links are read from HBM
+ * once per sweep instead of once per half-sweep (the second read comes from
+ * the Infinity Cache).  Every site sees the neighbour values of the
+ * two-kernel sweep: results are bit-identical either way.  The slab is
+ * chosen per lattice shape, gauge and mode by timing a few sweeps with each
+ * candidate on the first run of that kind in the process.
+ * gf_set_wavefront: -1 choose by timing (default), 0 never, n > 0 slabs of
+ * n timeslices; *previous (optional) receives the old setting.
+ * gf_wavefront_choice: the slab in use for gf_run_mixed of lat with `gauge`
+ * and gauge-sweep `mode` (0 two kernels, -1 not chosen yet).
+ * No reference counterpart. */
+gf_status gf_set_wavefront(int slab, int *previous);
+gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab);
 
 /* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
  * FP32 phase accumulates each site's gauge transformation g(x) until

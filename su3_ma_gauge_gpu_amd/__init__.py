@@ -46,6 +46,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     SR,
     U1XU1,
     UPDATE_DP,
+    SA_GRAPH,
     FILE_PLAIN,
     FILE_HEADERONLY,
     FILE_VOGT,
@@ -433,17 +434,18 @@ def run_fast_noalloc(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.
 
 def run_sa(lattice: Lattice, U, gauge: int = LANDAU, tmax: float = 0.4, tmin: float = 0.01, steps: int = 1000,
            micro: int = 3, reproject: int = 100, seed: int = 1, counter: int = 0, Udw=None, mode: int = 0,
-           stream=None) -> tuple[int, int]:
+           stream=None, graph: bool = False) -> tuple[int, int]:
     """Simulated annealing with the reference apps' schedule for ``gauge``
     (gf_run_sa in include/su3gf.h); mode 0 sweeps U itself, mode 1 anneals
     the gauge transformation in FP32 and applies it to U, mode 2 does the
-    same on a copy of U in 24-bit fixed point.  Returns (next RNG counter,
-    number of temperatures visited)."""
+    same on a copy of U in 24-bit fixed point; ``graph``: the schedule's
+    launches are captured into one HIP graph and replayed.  Returns (next RNG
+    counter, number of temperatures visited)."""
     ctr = C.c_uint32(counter)
     ntemps = C.c_int()
     _lib.call("gf_run_sa", C.byref(lattice.struct), _ptr(U), _ptr(Udw), gauge, C.c_float(tmax), C.c_float(tmin),
-              int(steps), int(micro), int(reproject), int(mode), int(seed), C.byref(ctr), C.byref(ntemps),
-              _ptr(stream))
+              int(steps), int(micro), int(reproject), int(mode) | (SA_GRAPH if graph else 0), int(seed),
+              C.byref(ctr), C.byref(ntemps), _ptr(stream))
     return int(ctr.value), int(ntemps.value)
 
 
@@ -463,7 +465,7 @@ def _batch_args(lattices, Us, streams, max_concurrent):
 
 def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: float = 0.01, steps: int = 1000,
                  micro: int = 3, reproject: int = 100, seeds=None, counters=None, mode: int = 0,
-                 max_concurrent: int = 4, streams=None) -> list[tuple[int, int]]:
+                 max_concurrent: int = 4, streams=None, graph: bool = False) -> list[tuple[int, int]]:
     """run_sa on independent 4D lattices in one call (gf_batch_run_sa in
     include/su3gf.h): small lattices concurrently, each on its own stream
     (``streams``: the caller's, one per concurrent problem, or None).
@@ -479,7 +481,7 @@ def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: flo
     ctr = (C.c_uint32 * n)(*counters)
     temps = (C.c_int * n)()
     _lib.call("gf_batch_run_sa", n, lats, ptrs, gauge, C.c_float(tmax), C.c_float(tmin), int(steps), int(micro),
-              int(reproject), int(mode), cs, ctr, temps, int(max_concurrent), strs)
+              int(reproject), int(mode) | (SA_GRAPH if graph else 0), cs, ctr, temps, int(max_concurrent), strs)
     return [(int(ctr[k]), int(temps[k])) for k in range(n)]
 
 

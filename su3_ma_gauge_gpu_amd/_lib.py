@@ -14,6 +14,7 @@ GF_OK, GF_ERR_BAD_SHAPE, GF_ERR_BAD_ARG, GF_ERR_HIP, GF_ERR_NOT_CONVERGED, GF_ER
 LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
 UPDATE_DP = 0x100  # OR'ed into an update rule: computed in double on FP32 links
+SA_GRAPH = 0x10  # OR'ed into an annealing mode: the schedule captured into one HIP graph
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
 MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24 = 0, 1, 2, 3
@@ -101,6 +102,8 @@ SIGNATURES = {
     "gf_run_fast": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_run_fast_noalloc": (_i, [_pl, _vp, _i, _f, _f, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
+    "gf_set_wavefront": (_i, [_i, C.POINTER(_i)]),
+    "gf_wavefront_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_batch_run_mixed": (_i, [_i, _pl, C.POINTER(_vp), _i, _f, _f, _f, _i, _i, _i, _i, _i, _i, C.POINTER(_vp),

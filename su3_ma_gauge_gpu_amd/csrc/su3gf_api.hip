@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <array>
+#include <map>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -595,6 +597,119 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
 
 }  // extern "C"
 
+// ------------------------------------------- wavefront slab autotuner
+// The gauge-transformation sweep can run a whole sweep as a wavefront over
+// timeslices (launch_gsweep_wavefront), so that a timeslice's links come from
+// HBM once per sweep instead of once per half-sweep; it pays only when the
+// lattice does not fit the Infinity Cache, and the best slab depends on the
+// lattice.  The first sweep of a (shape, gauge, link form, arithmetic) in the
+// process times a few OR sweeps with each candidate slab -- and with the two
+// kernels -- on a scratch copy of g and keeps the fastest; every choice gives
+// bit-identical results, so only the time depends on it.
+namespace {
+std::mutex g_slab_mu;
+std::map<std::array<int, 8>, int> g_slab_cache;
+int g_slab_setting = -1;  // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
+
+std::array<int, 8> slab_key(const Geom &geo, int gauge, int links24, bool fp64)
+{
+    return {geo.N[0], geo.N[1], geo.N[2], geo.N[3], gauge, links24, fp64 ? 1 : 0, geo.slice_bps ? 1 : 0};
+}
+
+// average ms of one sweep over `n` sweeps (after one unmeasured), slab 0 = two kernels
+float time_gsweep(const GaugeSweepArgs &t, int gauge, bool fp64, int slab, int n, hipStream_t s, hipEvent_t e0,
+                  hipEvent_t e1)
+{
+    auto sweep = [&]() -> hipError_t {
+        if (slab > 0) return launch_gsweep_wavefront(gauge, GF_OR, fp64, t, slab, s);
+        for (int par = 0; par < 2; par++) {
+            GaugeSweepArgs b = t;
+            b.parity = par;
+            const hipError_t e = launch_gsweep(gauge, GF_OR, 0, fp64, b, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    if (sweep() != hipSuccess) return -1.f;
+    if (hipEventRecord(e0, s) != hipSuccess) return -1.f;
+    for (int k = 0; k < n; k++)
+        if (sweep() != hipSuccess) return -1.f;
+    float ms = 0;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        return -1.f;
+    return ms / n;
+}
+
+// the slab for sweeps without a check of the gauge-transformation sweep
+// described by ga (0: the two-kernel sweep); never fails (0 on any error)
+int wavefront_slab(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
+{
+    const Geom &geo = ga.geo;
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (g_slab_setting >= 0) return g_slab_setting;
+    if (geo.N[0] < 4) return 0;
+    const auto key = slab_key(geo, gauge, ga.links24, fp64);
+    const auto hit = g_slab_cache.find(key);
+    if (hit != g_slab_cache.end()) return hit->second;
+    int best = 0;
+    {
+        Scratch g;
+        EventPair ev;
+        const size_t gbytes = sizeof(float) * gauge_floats(geo.V);
+        if (g.alloc(gbytes, s) == hipSuccess && hipMemcpyAsync(g.p, ga.g, gbytes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+            hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
+            GaugeSweepArgs t = ga;
+            t.g = g.as<float>();
+            t.state = nullptr;
+            t.partial = nullptr;
+            t.param = 1.7f;
+            const float two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+            float best_ms = two;
+            for (int c : {2, 3, 4, 6, 8, 12}) {
+                if (c >= geo.N[0] || two <= 0) break;
+                const float ms = time_gsweep(t, gauge, fp64, c, 3, s, ev.a, ev.b);
+                // a wavefront must beat the two kernels by 3% (launch count, noise)
+                if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
+                    best_ms = ms;
+                    best = c;
+                }
+            }
+        }
+        (void)hipStreamSynchronize(s);
+        (void)hipGetLastError();
+    }
+    g_slab_cache[key] = best;
+    return best;
+}
+}  // namespace
+
+extern "C" gf_status gf_set_wavefront(int slab, int *previous)
+{
+    if (slab < -1) return fail(GF_ERR_BAD_ARG, "wavefront setting %d (-1 tuned, 0 never, n > 0 fixed)", slab);
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (previous) *previous = g_slab_setting;
+    g_slab_setting = slab;
+    return GF_OK;
+}
+
+extern "C" gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab)
+{
+    int64_t V;
+    GF_TRY(check_lattice(lat, &V));
+    if (!slab) return fail(GF_ERR_BAD_ARG, "slab is null");
+    if (mode < GF_MIXED_GAUGE32 || mode > GF_MIXED_GAUGE24) return fail(GF_ERR_BAD_ARG, "not a gauge-sweep mode: %d", mode);
+    const Geom geo = per_slice(make_geom(lat->dims, V), gauge);
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (g_slab_setting >= 0) {
+        *slab = g_slab_setting;
+        return GF_OK;
+    }
+    const auto hit = g_slab_cache.find(slab_key(geo, gauge, mode == GF_MIXED_GAUGE24, mode == GF_MIXED_GAUGE64));
+    *slab = hit == g_slab_cache.end() ? -1 : hit->second;
+    return GF_OK;
+}
+
 // ------------------------------------------------------ simulated annealing
 extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
                                int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
@@ -605,6 +720,8 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     if (!U || !counter) return fail(GF_ERR_BAD_ARG, "null argument");
     if (steps < 0 || micro < 0) return fail(GF_ERR_BAD_ARG, "negative step count");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
+    const bool graph = (mode & GF_SA_GRAPH) != 0;
+    mode &= ~GF_SA_GRAPH;
     if (mode < 0 || mode > 2) return fail(GF_ERR_BAD_ARG, "unknown annealing mode %d", mode);
     if (mode > 0) GF_TRY(check_gauge_sweep_volume(V));
     const bool gform = mode != 0;  // the gauge transformation is annealed (modes 1, 2)
@@ -644,19 +761,29 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     };
     // one heatbath temperature: SA on both parities, then `micro`
     // microcanonical pairs (LandauGaugeFixingSU3_4D.cu:214-223)
+    // modes 1, 2: a sweep as a wavefront over timeslices where that pays
+    const int slab = gform ? wavefront_slab(ga, gauge, false, s) : 0;
+    auto gsweep_pair = [&](int algo) -> hipError_t {
+        if (slab > 0) {
+            const hipError_t e = launch_gsweep_wavefront(gauge, algo, false, ga, slab, s);
+            if (e != hipErrorNotSupported) return e;
+        }
+        for (int par = 0; par < 2; par++) {
+            GaugeSweepArgs b = ga;
+            b.parity = par;
+            b.counter = ga.counter + (uint32_t)par;
+            const hipError_t e = launch_gsweep(gauge, algo, 0, false, b, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
     auto heat = [&](float T) -> gf_status {
         if (gform) {
-            for (int par = 0; par < 2; par++) {
-                ga.parity = par;
-                ga.param = T;
-                ga.counter = (*counter)++;
-                HIP_TRY(launch_gsweep(gauge, GF_SA, 0, false, ga, s));
-            }
-            for (int m = 0; m < micro; m++)
-                for (int par = 0; par < 2; par++) {
-                    ga.parity = par;
-                    HIP_TRY(launch_gsweep(gauge, GF_MICRO, 0, false, ga, s));
-                }
+            ga.param = T;
+            ga.counter = *counter;
+            HIP_TRY(gsweep_pair(GF_SA));
+            *counter += 2;
+            for (int m = 0; m < micro; m++) HIP_TRY(gsweep_pair(GF_MICRO));
             return GF_OK;
         }
         for (int par = 0; par < 2; par++) GF_TRY(gf_sweep(lat, U, dw, par, gauge, GF_SA, T, seed, (*counter)++, 32, s));
@@ -664,6 +791,38 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
             for (int par = 0; par < 2; par++) GF_TRY(gf_sweep(lat, U, dw, par, gauge, GF_MICRO, 0.f, 0, 0, 32, s));
         return GF_OK;
     };
+    // graph mode: the schedule's launches are captured on `cs` (the caller's
+    // stream, or a blocking stream of our own that follows the null stream)
+    // and replayed once on it
+    hipStream_t cs = s;
+    struct GraphRun {
+        hipStream_t own = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        bool capturing = false;
+        hipStream_t cs = nullptr;
+        ~GraphRun()
+        {
+            if (capturing) {
+                hipGraph_t dropped = nullptr;
+                (void)hipStreamEndCapture(cs, &dropped);
+                if (dropped) (void)hipGraphDestroy(dropped);
+            }
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            if (own) (void)hipStreamDestroy(own);
+        }
+    } gr;
+    if (graph) {
+        if (!cs) {
+            HIP_TRY(hipStreamCreate(&gr.own));
+            cs = gr.own;
+        }
+        gr.cs = cs;
+        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        gr.capturing = true;
+        s = cs;  // every launch below is captured
+    }
     float temperature = tmax;
     const float base = (tmax - tmin) / (float)steps;
     int n = 0;
@@ -691,6 +850,13 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
             i++;
             n++;
         } while (temperature >= tmin);
+    }
+    if (graph) {
+        gr.capturing = false;
+        HIP_TRY(hipStreamEndCapture(cs, &gr.g));
+        HIP_TRY(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+        HIP_TRY(hipGraphLaunch(gr.x, cs));
+        if (gr.own) HIP_TRY(hipStreamSynchronize(gr.own));
     }
     if (gform) {
         // U <- G U G^dagger in FP64 (G the FP64 projection of g), third rows rebuilt
@@ -790,10 +956,13 @@ constexpr double kCoarseFactor = 1e3;
 constexpr double kNearFactor = 2.0;
 constexpr double kHostNearFactor = 1e3;
 
-template <typename Launch, typename After>
+// plain(it): enqueue all of sweep `it` (no check involved) at once and return
+// hipSuccess, or hipErrorNotSupported to have its two parity kernels launched
+template <typename Launch, typename After, typename Plain>
 gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int maxiter, int check_every,
                      int coarse_every, int stall_limit, bool exact_end, bool near, int64_t nblocks, FusedCtx &fc,
-                     hipStream_t s, Launch launch, After after_sweep, int *sweeps_done, gf_run_result *r)
+                     hipStream_t s, Launch launch, After after_sweep, Plain plain, int *sweeps_done,
+                     gf_run_result *r)
 {
     const int nseg = check_segments(dims, gauge);
     if (check_every < 1) check_every = 1;
@@ -818,6 +987,15 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
             const bool check_prev = pending;                    // end of sweep it-1
             const bool check_this = (it + 1) % every == 0;      // end of sweep it
             const bool pre = check_prev && host_near;
+            if (!check_prev && !check_this) {
+                e = plain(it);
+                if (e == hipSuccess) {
+                    pending = false;
+                    GF_TRY(after_sweep(it));
+                    continue;
+                }
+                if (e != hipErrorNotSupported) return hip_fail(e, "sweep launch");
+            }
             if (pre) {
                 // read-only theta of the even sites (a no-op until `near`), its
                 // check, then the even update, which exits if that converged
@@ -875,8 +1053,9 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
         a.partial = partial;
         return fast::launch_sweep<R>(lat->layout, gauge, GF_OR, theta, gacc != nullptr, a, s);
     };
+    auto two_kernels = [](int) { return hipErrorNotSupported; };
     return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit, exact_end, near,
-                      sweep_grid(a.g), fc, s, launch, after_sweep, sweeps_done, r);
+                      sweep_grid(a.g), fc, s, launch, after_sweep, two_kernels, sweeps_done, r);
 }
 
 // projectSU3 enqueued by a driver after a sweep: skipped on the device once
@@ -1006,12 +1185,13 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
-    if (variant < 0 || variant > 5) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
+    if (variant < 0 || variant > 6) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
     if (gauge == GF_COULOMB && lat->dims[0] == 1) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D lattices");
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
-    if (variant > 0 && variant < 5 && lat->dtype != GF_F32) return fail(GF_ERR_BAD_ARG, "variants 1-4 time FP32 kernels");
-    if (variant >= 2 && variant < 5) GF_TRY(check_gauge_sweep_volume(V));
+    if (variant > 0 && variant != 5 && lat->dtype != GF_F32)
+        return fail(GF_ERR_BAD_ARG, "variants 1-4 and 6 time FP32 kernels");
+    if (variant >= 2 && variant != 5) GF_TRY(check_gauge_sweep_volume(V));
     hipStream_t s = S(stream);
     if (variant == 5) {
         // the drop-in quality kernel (GaugeFixingStats::generateGaugeQuality's
@@ -1049,7 +1229,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         gb.p = gbuf.as<float>();
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
     }
-    if (variant == 4) {  // the links in the 24-bit fixed-point form
+    if (variant == 4 || variant == 6) {  // the links in the 24-bit fixed-point form
         HIP_TRY(pbuf.alloc(sizeof(uint32_t) * links24_words(V), s));
         HIP_TRY(launch_pack24(GF_F32, 12, U, pbuf.as<uint32_t>(), V, s));
     }
@@ -1059,8 +1239,27 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
     const Geom geo = make_geom(lat->dims, V);
+    int slab = 0;
+    GaugeSweepArgs wa{};  // variant 6: whole sweeps as the mixed run's FP32 phase runs them
+    if (variant == 6) {
+        wa.U0 = pbuf.as<const float>(); wa.g = gb.p; wa.geo = geo; wa.param = 1.7f; wa.remap = 1; wa.links24 = 1;
+        slab = wavefront_slab(wa, gauge, false, s);
+    }
     auto run = [&](int n, int first) -> gf_status {
         for (int it = 0; it < n; it++) {
+            if (variant == 6) {
+                hipError_t e = slab > 0 ? launch_gsweep_wavefront(gauge, GF_OR, false, wa, slab, s) : hipErrorNotSupported;
+                if (e == hipErrorNotSupported) {  // the two kernels
+                    e = hipSuccess;
+                    for (int par = 0; par < 2 && e == hipSuccess; par++) {
+                        GaugeSweepArgs b = wa;
+                        b.parity = par;
+                        e = launch_gsweep(gauge, GF_OR, 0, false, b, s);
+                    }
+                }
+                HIP_TRY(e);
+                continue;
+            }
             for (int par = 0; par < 2; par++) {
                 double *partial = par ? fc.odd() : fc.even();
                 if (lat->dtype == GF_F64) {
@@ -1185,8 +1384,15 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
             return launch_gsweep(gauge, GF_OR, theta, mode == GF_MIXED_GAUGE64, ga, s);
         };
         auto none = [](int) -> gf_status { return GF_OK; };
+        // sweeps without a check: as a wavefront over timeslices where that
+        // pays (lattices beyond the Infinity Cache)
+        const int slab = wavefront_slab(ga, gauge, mode == GF_MIXED_GAUGE64, s);
+        auto plain = [&](int) {
+            if (slab <= 0) return hipErrorNotSupported;
+            return launch_gsweep_wavefront(gauge, GF_OR, mode == GF_MIXED_GAUGE64, ga, slab, s);
+        };
         GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kCoarse * check32, kStall32,
-                          false, false, sweep_grid(geo), fc, s, launch, none, &done32, out));
+                          false, false, sweep_grid(geo), fc, s, launch, none, plain, &done32, out));
     }
     out->sweeps_fp32 = done32;
     // switch: apply the accumulated transformation to the FP64 configuration,

@@ -38,6 +38,7 @@
 // Delta = sum U_up - sum U_down = g P^dagger - Q g^dagger and the functional
 // Re tr of every link (each link has exactly one endpoint of either parity).
 // These are the quantities of LandauKernelsSU3.hxx:114-191.
+#include <algorithm>
 #include <type_traits>
 
 #include "su3gf_sweep.h"
@@ -391,6 +392,27 @@ __device__ __forceinline__ int64_t logical_block(const GaugeSweepArgs &a)
     return t * a.slice_blocks + q;
 }
 
+// Wavefront launch: the parity and logical block of this block, or false for
+// a padding block (uniform over the block)
+__device__ __forceinline__ bool wave_block(const GaugeSweepArgs &a, int64_t &blk, int &parity)
+{
+    int64_t b = blockIdx.x, n;
+    if (b < a.wf_even_grid) {
+        parity = 0;
+        n = a.wf_even_n;
+    } else {
+        parity = 1;
+        b -= a.wf_even_grid;
+        n = a.wf_odd_n + a.wf_odd1_n;
+    }
+    const int64_t per = (n + 7) / 8;
+    const int64_t k = (b & 7) * per + (b >> 3);
+    if (k >= n) return false;
+    if (parity == 0) blk = a.wf_even0 + k;
+    else blk = k < a.wf_odd_n ? a.wf_odd0 + k : a.wf_odd1 + (k - a.wf_odd_n);
+    return true;
+}
+
 // The update rule on the subgroup quaternion, with the U1xU1 restriction to
 // (q0, q3) and normalisation (GaugeFixingSubgroupStep.hxx:320-338).
 template <typename A, int G, int ALGO>
@@ -504,10 +526,10 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
 // Philox generator of a site (SA, random transformation): key word 0 = global
 // id of the reference's (mu = 0, up) thread of this site in its V/2/NSB x
 // 8*NSB launch (PhiloxWrapper.hxx:93-110)
-__device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, Philox &rng)
+__device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, int parity, Philox &rng)
 {
     const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
-    philox_init(rng, tid, a.seed, a.counter);
+    philox_init(rng, tid, a.seed, a.wf && parity ? a.counter_odd : a.counter);
 }
 
 // Work mapping 0: a block covers 64 sites with one wave per direction of S
@@ -523,10 +545,16 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mu = wave + (4 - kW);  // one direction per wave
     const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a);
+    int parity = a.parity;
+    int64_t blk;
+    if (a.wf) {
+        if (!wave_block(a, blk, parity)) return;
+    } else {
+        blk = logical_block(a);
+    }
     bool active;
     const int64_t p = block_site(a.geo, blk, lane, active);
-    const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
+    const int64_t site = (active ? p : 0) + (parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
     const uint32_t site_off = (uint32_t)site * 4u;
@@ -536,7 +564,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     if (wave == 0) load_rows01(fg, tile_off(site_off, 12), Gx);
 
     Philox rng;
-    if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, rng);
+    if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, parity, rng);
 
     if (ALGO == GF_RANDOM) {
         // the random transformation does not depend on the links
@@ -598,10 +626,16 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     if (a.state && a.state->done) return;  // converged: uniform early exit
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int64_t blk = logical_block(a);
+    int parity = a.parity;
+    int64_t blk;
+    if (a.wf) {
+        if (!wave_block(a, blk, parity)) return;
+    } else {
+        blk = logical_block(a);
+    }
     bool active;
     const int64_t p = block_site(a.geo, blk, lane, active);
-    const int64_t site = (active ? p : 0) + (a.parity ? a.geo.Vh : 0);
+    const int64_t site = (active ? p : 0) + (parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
     const uint32_t site_off = (uint32_t)site * 4u;
@@ -611,7 +645,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     load_rows01(fg, tile_off(site_off, 12), Gx);
     third_row(Gx);
     Philox rng;  // the same stream in every wave
-    if (ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) site_rng(a, p, rng);
+    if (ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) site_rng(a, p, parity, rng);
 
     M3<A> up, dn;
     {
@@ -692,9 +726,16 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     }
 }
 
+// blocks of a launch: one per logical block, or a wavefront launch's even and odd grids
+inline int64_t gsweep_grid(const GaugeSweepArgs &a)
+{
+    if (!a.wf) return sweep_grid(a.geo);
+    return a.wf_even_grid + 8 * ((a.wf_odd_n + a.wf_odd1_n + 7) / 8);
+}
+
 template <typename A, int ALGO, int THETA, int LF> void launch_k_mag(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)sweep_grid(a.geo)), block(kGBlock);
+    const dim3 grid((unsigned)gsweep_grid(a)), block(kGBlock);
     hipLaunchKernelGGL((k_gsweep_mag<A, ALGO, THETA, LF>), grid, block, 0, s, a);
 }
 
@@ -723,7 +764,7 @@ template <typename A, int LF> hipError_t launch_mag_lf(int algo, int theta, cons
 template <typename A, int G, int ALGO, int THETA, int LF>
 void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)sweep_grid(a.geo)), block(64 * gsweep_waves<G>());
+    const dim3 grid((unsigned)gsweep_grid(a)), block(64 * gsweep_waves<G>());
     hipLaunchKernelGGL((k_gsweep<A, G, ALGO, THETA, LF>), grid, block, 0, s, a);
 }
 
@@ -863,7 +904,50 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
         b.remap = 0;
     }
     if (b.nsb <= 0) b.nsb = 32;
+    b.wf = 0;
     return fp64_arith ? launch_a<double>(gauge, algo, theta, b, s) : launch_a<float>(gauge, algo, theta, b, s);
+}
+
+hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, int slab,
+                                   hipStream_t s)
+{
+    if (!gauge_sweep_fits(a.geo.V)) return hipErrorInvalidValue;
+    const int N = a.geo.N[0];
+    const int64_t grid = sweep_grid(a.geo);
+    // blocks per timeslice: whole 64-site blocks, or a Coulomb lattice's
+    // per-slice blocks (each slice's last block padded)
+    if (N < 4 || slab < 1 || grid % N != 0 || (!a.geo.slice_bps && (a.geo.Vh / N) % 64 != 0))
+        return hipErrorNotSupported;
+    const int64_t bps = grid / N;
+    const int K = (N + slab - 1) / slab;
+    auto e = [&](int k) { return (int64_t)std::min(k * slab, N); };
+    GaugeSweepArgs b = a;
+    if (b.nsb <= 0) b.nsb = 32;
+    b.wf = 1;
+    b.parity = 0;
+    b.counter_odd = a.counter + 1;
+    for (int k = 0; k <= K; k++) {
+        // even timeslices [e(k), e(k+1)); odd timeslices [max(1, e(k-1) - 1), e(k) - 1),
+        // and in the last launch the remaining ones and timeslice 0
+        const int64_t even_lo = k < K ? e(k) : 0, even_hi = k < K ? e(k + 1) : 0;
+        int64_t odd_lo = 0, odd_hi = 0;
+        if (k > 0) {
+            odd_lo = std::max<int64_t>(1, e(k - 1) - 1);
+            odd_hi = k < K ? e(k) - 1 : N;
+            if (odd_hi < odd_lo) odd_hi = odd_lo;
+        }
+        b.wf_even0 = even_lo * bps;
+        b.wf_even_n = (even_hi - even_lo) * bps;
+        b.wf_even_grid = 8 * ((b.wf_even_n + 7) / 8);
+        b.wf_odd0 = odd_lo * bps;
+        b.wf_odd_n = (odd_hi - odd_lo) * bps;
+        b.wf_odd1 = 0;
+        b.wf_odd1_n = k == K ? bps : 0;
+        if (gsweep_grid(b) == 0) continue;
+        const hipError_t err = fp64_arith ? launch_a<double>(gauge, algo, 0, b, s) : launch_a<float>(gauge, algo, 0, b, s);
+        if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
 }
 
 }  // namespace su3gf

@@ -89,11 +89,35 @@ struct GaugeSweepArgs {
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
     int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR, SA, microcanonical)
+    // Wavefront launch (wf = 1, set by launch_gsweep_wavefront): one launch
+    // sweeps the even sites of logical blocks [wf_even0, wf_even0 + wf_even_n)
+    // with its first wf_even_grid blocks and the odd sites of
+    // [wf_odd0, wf_odd0 + wf_odd_n), then of [wf_odd1, wf_odd1 + wf_odd1_n),
+    // with the rest; each range is dealt to the XCDs in contiguous eighths
+    // (grids padded to multiples of 8, padding blocks idle).  Odd sites draw
+    // their random numbers with counter_odd.
+    int wf;
+    int64_t wf_even0, wf_even_n, wf_even_grid;
+    int64_t wf_odd0, wf_odd_n, wf_odd1, wf_odd1_n;
+    uint32_t counter_odd;
 };
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
 // 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
 // compute in FP64 on the FP32 data
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+// One whole sweep (even then odd sites, no fused check) of a 4D lattice as a
+// wavefront over timeslices: launch k updates the even sites of timeslices
+// [kS, (k+1)S) and the odd sites of [(k-1)S - 1, kS - 1) (slab = S; slice 0's
+// and the last slices' odd sites in a final launch), so every site sees the
+// neighbour values of the two-kernel sweep -- the results are bit-identical
+// -- while a timeslice's links, read by its even update, are read again by
+// its odd update one launch later, from the Infinity Cache instead of HBM.
+// The odd sites use counter a.counter + 1 (the two-kernel sweep's odd
+// launch).  Returns hipErrorNotSupported where the wavefront does not apply
+// (fewer than 4 timeslices, or a timeslice's sites of a parity not whole
+// 64-site blocks); the caller then launches the two kernels.
+hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, int slab,
+                                   hipStream_t s);
 // words of the 24-bit link copy: 9 per link, in whole tiles of 64 sites
 __host__ __device__ inline int64_t links24_words(int64_t V) { return 4 * 9 * 64 * ((V + 63) / 64); }
 // links of dtype ti and layout si (FP64 18 or 12, FP32 12) -> the 24-bit

@@ -6,12 +6,20 @@ for kernel traces and hardware-counter runs:
 * `--sweeps` FP32 gauge-transformation sweeps on read-only compact FP32
   links, FP32 arithmetic (the mixed driver's default FP32 kernel), the
   same with FP64 arithmetic, and on the links in 24-bit fixed point;
+* `--sweeps` whole sweeps of the 24-bit kernel as the mixed driver runs a
+  sweep without a check: a wavefront over timeslices, with the slab the
+  library's timing picks for this lattice (or the two kernels);
 * `--sweeps` FP32 link sweeps with gauge accumulation on the compact FP32
   layout (the mixed driver's link mode);
-* `--sweeps` fused-check FP64 sweeps on the 18-real layout (the FP64 kernel);
-* `--sweeps` drop-in (reference-order) FP64 sweeps through gf_sweep.
+* `--sweeps` fused-check FP64 sweeps on the drivers' tiled copy of the links
+  (the FP64 kernel of gf_run_fast and of the mixed run's FP64 phase);
+* `--sweeps` drop-in (reference-order) FP64 sweeps through gf_sweep on the
+  18-real GpuPattern buffer;
+* `--sweeps` launches of the drop-in quality kernel (GaugeFixingStats'
+  per-site pass) on the same buffer.
 
 Usage: python su3_ma_gauge_gpu_amd/profile_sweeps.py [--dims 32 32 32 32] [--sweeps 10]
+(--dims 48 48 48 48: a working set far beyond the 256 MB Infinity Cache)
 """
 from __future__ import annotations
 
@@ -46,19 +54,23 @@ def main() -> None:
     ms = C.c_float()
     for variant, what in ((2, "fp32 gauge-transformation sweep"), (3, "fp32 gauge-transformation sweep, fp64 arithmetic"),
                           (4, "fp32 gauge-transformation sweep, 24-bit fixed-point links"),
+                          (6, "the same, each sweep as the mixed run's FP32 phase runs it (per half sweep)"),
                           (1, "fp32 link sweep with gauge accumulation")):
         _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, variant,
                   C.byref(ms), None)
         print(f"{what}: {ms.value:.4f} ms per kernel", flush=True)
     _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 0, C.byref(ms),
               None)
-    print(f"fp64 fused sweep: {ms.value:.4f} ms per kernel", flush=True)
+    print(f"fp64 fused sweep (tiled copy): {ms.value:.4f} ms per kernel", flush=True)
     K = gf.LandauKernelsSU3(lat64)
     for _ in range(args.sweeps):
         for parity in (0, 1):
             K.orStep(lat64.V // 2 // 32, 256, work, None, parity, 1.7)
     gf.synchronize()
     print("drop-in sweeps done", flush=True)
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 5, C.byref(ms),
+              None)
+    print(f"drop-in quality: {ms.value:.4f} ms per kernel", flush=True)
 
 
 if __name__ == "__main__":

@@ -124,3 +124,48 @@ def test_gauge_copies_full_schedule(gf, gpu, L, copies):
         p_in = _plaquette(to_matrices(u, V), dims)
         p_out = _plaquette(to_matrices(b.download(np.float64, u.size), V), dims)
         assert p_out == pytest.approx(p_in, rel=1e-12)
+
+
+def test_mag_48_bounded(gf, gpu):
+    """The MAG workload's 48^4 lattice (sigma 0.35, seed 1, Haar seed 2) over
+    a bounded number of sweeps (the full run needs some 10^5): 200 mixed
+    sweeps (all in the FP32 phase, on 24-bit links) and 200 FP64 sweeps raise
+    MAG's functional by the same amount to FP32 accuracy, the mixed result is
+    a gauge transformation of the input (the plaquette of every 97th site to
+    1e-12), and 50 U1xU1 sweeps on the FP64 result keep MAG's functional and
+    theta (diagonal gauge rotations)."""
+    dims = (48, 48, 48, 48)
+    V = 48 ** 4
+    print("generating the 48^4 configuration", flush=True)
+    U = gf.make_config(dims, 0.35, 1, 2)
+    lat = gf.Lattice(dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+
+    def mag_q(buf):
+        st = gf.GaugeFixingStats(gf.MAGKernelsSU3, buf, lat)
+        st.generateGaugeQuality()
+        out = (st.getCurrentGff(), st.getCurrentA())
+        st.close()
+        return out
+
+    g0, t0 = mag_q(d)
+    m = gf.run_mixed(lat, d, gf.MAG, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=200, check32=5)
+    print("mixed", m, flush=True)
+    assert m.sweeps == m.sweeps_fp32 == 200 and not m.converged
+    d64 = gf.DeviceBuffer.from_numpy(U)
+    f = gf.run_fast(lat, d64, gf.MAG, 1.7, 0.0, 200, 100)
+    print("fp64", f, flush=True)
+    assert f.sweeps == 200 and not f.converged
+    gm, tm = mag_q(d)
+    gf64, tf64 = mag_q(d64)
+    assert gm > g0 + 0.1 and gf64 > g0 + 0.1
+    assert gm == pytest.approx(gf64, rel=1e-6)
+    assert tm == pytest.approx(tf64, rel=1e-2)
+    p_in = _plaquette(to_matrices(U, V), dims, every=97)
+    p_out = _plaquette(to_matrices(d.download(np.float64, U.size), V), dims, every=97)
+    assert p_out == pytest.approx(p_in, rel=1e-12)
+    u = gf.run_fast(lat, d64, gf.U1XU1, 1.7, 0.0, 50, 100)
+    g1, t1 = mag_q(d64)
+    print("u1xu1", u, g1, t1, flush=True)
+    assert g1 == pytest.approx(gf64, rel=1e-12)
+    assert t1 == pytest.approx(tf64, rel=1e-6)

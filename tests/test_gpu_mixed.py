@@ -414,3 +414,121 @@ def test_unconverged_coulomb_reports_largest_slice_theta(gf, gpu, oracle, driver
     assert res.theta == pytest.approx(max(t for _, t in q), rel=1e-9)
     assert abs(res.theta - np.mean([t for _, t in q])) > 1e-3 * res.theta
     assert res.gff == pytest.approx(np.mean([g for g, _ in q]), rel=1e-12)
+
+
+def test_u1xu1_on_mag_result(gf, gpu, oracle):
+    """The MAG workload's chain on a small lattice (sigma 0.35): MAG fixed in
+    mixed precision to theta < 1e-10 (the oracle's maximum, functional to
+    1e-10), then 100 U1xU1 OR sweeps on the result by the fused FP64 driver
+    and by the oracle from the same start (both reproject at sweep 0): the
+    U1xU1 functionals agree to 1e-10, and neither run changes MAG's
+    functional or theta, since a diagonal gauge rotation changes neither
+    |U_jj| nor |X_k|."""
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims, sigma=0.35)
+    lat = gf.Lattice(dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    m = gf.run_mixed(lat, d, gf.MAG, 1.7, switch_precision=1e-6, precision=1e-10, maxiter=50000, check32=5)
+    n, g_or, _ = oracle.run_or(MAG, dims, U.copy(), 1.7, 1e-10, 50000, 1, 100)
+    assert m.converged == 1 and m.gff == pytest.approx(g_or, rel=1e-10)
+    R = d.download()
+    g_mag, t_mag = oracle.quality(MAG, dims, R.copy())
+    assert t_mag < 1e-10
+    u = gf.run_fast(lat, d, gf.U1XU1, 1.7, 0.0, 100, 100)
+    Ro = R.copy()
+    no, g_uo, _ = oracle.run_or(U1XU1, dims, Ro, 1.7, 0.0, 100, 1, 100)
+    assert u.sweeps == no == 100 and not u.converged
+    assert u.gff == pytest.approx(g_uo, rel=1e-10)
+    g_u1_start = oracle.quality(U1XU1, dims, R.copy())[0]
+    assert u.gff > g_u1_start + 1e-4  # the U1xU1 sweeps did work
+    for after in (d.download(), Ro):
+        g2, t2 = oracle.quality(MAG, dims, after.copy())
+        assert g2 == pytest.approx(g_mag, rel=1e-12)
+        assert t2 == pytest.approx(t_mag, rel=1e-3, abs=1e-13)
+    del V
+
+
+@pytest.mark.parametrize("gauge,mode", [(LANDAU, 0), (LANDAU, 1), (LANDAU, 2), (MAG, 2), (U1XU1, 0)])
+def test_run_sa_graph_is_bit_identical(gf, gpu, gauge, mode):
+    """GF_SA_GRAPH: the annealing schedule captured into one HIP graph and
+    replayed gives the links, counters and temperature count of the same
+    schedule launched one kernel at a time -- on the default stream (the call
+    captures on a stream of its own) and on the caller's stream, and through
+    the batch call on the caller's streams."""
+    import ctypes as C
+
+    dims = (8, 8, 8, 8)
+    U = _cfg(gf, dims, sigma=0.35 if gauge == MAG else 0.3)
+    lat = gf.Lattice(dims)
+    kw = dict(tmax=0.9, tmin=0.6, steps=20) if gauge == MAG else dict(tmax=1.45, tmin=0.01, steps=40)
+    if gauge == U1XU1:
+        kw = dict(tmax=2.8, tmin=1.9, steps=10)
+    stream = C.c_void_p()
+    gf._lib.call("gf_stream_create", C.byref(stream))
+    out = {}
+    for graph, st in ((False, None), (True, None), (True, stream)):
+        d = gf.DeviceBuffer.from_numpy(U)
+        ctr = gf.run_sa(lat, d, gauge, micro=3, reproject=7, seed=5, counter=100, mode=mode, stream=st, graph=graph,
+                        **kw)
+        gf.synchronize()
+        out[(graph, st is not None)] = (ctr, d.download())
+    ref = out[(False, False)]
+    for k, (ctr, u) in out.items():
+        assert ctr == ref[0], k
+        assert np.array_equal(u, ref[1]), k
+    stream2 = C.c_void_p()
+    gf._lib.call("gf_stream_create", C.byref(stream2))
+    bufs = [gf.DeviceBuffer.from_numpy(U) for _ in range(2)]
+    res = gf.batch_run_sa([lat] * 2, bufs, gauge, micro=3, reproject=7, seeds=[5, 5], counters=[100, 100], mode=mode,
+                          max_concurrent=2, streams=[stream, stream2], graph=True, **kw)
+    gf.synchronize()
+    for r, b in zip(res, bufs):
+        assert r == ref[0]
+        assert np.array_equal(b.download(), ref[1])
+    for st in (stream, stream2):
+        gf._lib.call("gf_stream_destroy", st)
+
+
+def _set_wavefront(gf, slab):
+    import ctypes as C
+    prev = C.c_int()
+    gf._lib.call("gf_set_wavefront", int(slab), C.byref(prev))
+    return prev.value
+
+
+@pytest.mark.parametrize("gauge,dims", [(LANDAU, (8, 8, 8, 8)), (LANDAU, (12, 8, 8, 8)), (COULOMB, (8, 6, 4, 8)),
+                                        (MAG, (8, 8, 8, 8)), (U1XU1, (6, 8, 8, 8))])
+def test_wavefront_sweeps_are_bit_identical(gf, gpu, gauge, dims):
+    """A sweep run as a wavefront over timeslices (launch k: even sites of
+    slab k, odd sites of the slab before it shifted back by one timeslice,
+    timeslice 0's odd sites last) gives the two-kernel sweep's result bit for
+    bit, for every slab from one timeslice to more than the lattice: the mixed
+    driver's FP32 phase (24-bit and FP32 links; sweeps without a check) and
+    the gauge-annealing schedule (heatbath random numbers per site and
+    parity, microcanonical steps)."""
+    U = _cfg(gf, dims, sigma=0.35 if gauge == MAG else 0.3)
+    lat = gf.Lattice(dims)
+    prev = _set_wavefront(gf, 0)
+    try:
+        want = {}
+        for slab in (0, 1, 2, 3, 5, dims[0] + 1):
+            _set_wavefront(gf, slab)
+            got = {}
+            for mode in (gf.MIXED_GAUGE24, gf.MIXED_GAUGE32):
+                d = gf.DeviceBuffer.from_numpy(U)
+                r = gf.run_mixed(lat, d, gauge, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=23, mode=mode,
+                                 check32=5)
+                got[mode] = (r.sweeps, r.checks, d.download())
+            if gauge != COULOMB:
+                d = gf.DeviceBuffer.from_numpy(U)
+                ctr = gf.run_sa(lat, d, gauge, 1.2, 0.1, 6, 2, 100, 3, 40, mode=2)
+                got["sa"] = (ctr, d.download())
+            if slab == 0:
+                want = got
+                continue
+            for k, v in got.items():
+                assert v[:-1] == want[k][:-1], (slab, k)
+                assert np.array_equal(v[-1], want[k][-1]), (slab, k)
+    finally:
+        _set_wavefront(gf, prev)

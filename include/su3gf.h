@@ -339,6 +339,10 @@ gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *s
  *     (each real as round(v (2^23 - 1)), four to three 32-bit words): 36
  *     instead of 48 bytes per link, a quantisation error of at most 2^-24
  *     per real, the size of an FP32 rounding error of an entry near 1.
+ *   In GAUGE32 and GAUGE24, an FP32 phase that stagnates above
+ *   switch_precision continues on the same g and links with FP64
+ *   arithmetic (the GAUGE64 kernel, whose floor is lower) until
+ *   switch_precision or stagnation again; out->sweeps_fp32 counts both.
  * The gauge-transformation modes take at most 44,739,200 sites (80^4 fits;
  * GF_ERR_BAD_SHAPE beyond), GF_MIXED_LINKS any volume below 2^31.
  * No reference counterpart (the reference's DPUPDATES is a compile-time

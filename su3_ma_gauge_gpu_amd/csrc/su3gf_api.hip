@@ -611,9 +611,10 @@ std::mutex g_slab_mu;
 std::map<std::array<int, 8>, int> g_slab_cache;
 int g_slab_setting = -1;  // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
 
-std::array<int, 8> slab_key(const Geom &geo, int gauge, int links24, bool fp64)
+// links: 0 FP32, 1 24-bit, 2 FP64 (with an FP64 g)
+std::array<int, 8> slab_key(const Geom &geo, int gauge, int links, bool fp64)
 {
-    return {geo.N[0], geo.N[1], geo.N[2], geo.N[3], gauge, links24, fp64 ? 1 : 0, geo.slice_bps ? 1 : 0};
+    return {geo.N[0], geo.N[1], geo.N[2], geo.N[3], gauge, links, fp64 ? 1 : 0, geo.slice_bps ? 1 : 0};
 }
 
 // average ms of one sweep over `n` sweeps (after one unmeasured), slab 0 = two kernels
@@ -649,14 +650,14 @@ int wavefront_slab(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s
     std::lock_guard<std::mutex> lock(g_slab_mu);
     if (g_slab_setting >= 0) return g_slab_setting;
     if (geo.N[0] < 4) return 0;
-    const auto key = slab_key(geo, gauge, ga.links24, fp64);
+    const auto key = slab_key(geo, gauge, ga.links64 ? 2 : ga.links24, fp64);
     const auto hit = g_slab_cache.find(key);
     if (hit != g_slab_cache.end()) return hit->second;
     int best = 0;
     {
         Scratch g;
         EventPair ev;
-        const size_t gbytes = sizeof(float) * gauge_floats(geo.V);
+        const size_t gbytes = (ga.links64 ? sizeof(double) : sizeof(float)) * gauge_floats(geo.V);
         if (g.alloc(gbytes, s) == hipSuccess && hipMemcpyAsync(g.p, ga.g, gbytes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
             hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
             GaugeSweepArgs t = ga;
@@ -1325,9 +1326,14 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
     const Geom geo = per_slice(make_geom(lat->dims, V), gauge);
-    Scratch sU32, sg;
+    Scratch sU32, sg, sg64;
     HIP_TRY(sU32.alloc(mode == GF_MIXED_GAUGE24 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
     HIP_TRY(sg.alloc(sizeof(float) * gauge_floats(V), s));
+    // the gauge modes' FP64 phase sweeps an FP64 g on the FP64 links (the
+    // link mode, and lattices beyond the FP64 gauge sweep's offsets, sweep
+    // the links themselves)
+    const bool gauge64 = mode != GF_MIXED_LINKS && gauge_sweep64_fits(V);
+    if (gauge64) HIP_TRY(sg64.alloc(sizeof(double) * gauge_floats(V), s));
     struct {
         void *U32, *g;
     } b{sU32.p, sg.p};
@@ -1393,6 +1399,26 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         };
         GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter, check32, kCoarse * check32, kStall32,
                           false, false, sweep_grid(geo), fc, s, launch, none, plain, &done32, out));
+        if (out->converged == 2 && mode != GF_MIXED_GAUGE64 && done32 < maxiter) {
+            // FP32 arithmetic stagnated above switch_precision (its rounding
+            // floor, near theta ~ 2e-12 for Landau): the same g and links
+            // with FP64 arithmetic, whose floor is the FP32 storage's (~4e-13)
+            // -- a third of the cost of an FP64 link sweep
+            auto launch64 = [&](int parity, int theta, double *partial, int) {
+                ga.parity = parity;
+                ga.partial = partial;
+                return launch_gsweep(gauge, GF_OR, theta, true, ga, s);
+            };
+            const int slab64 = wavefront_slab(ga, gauge, true, s);
+            auto plain64 = [&](int) {
+                if (slab64 <= 0) return hipErrorNotSupported;
+                return launch_gsweep_wavefront(gauge, GF_OR, true, ga, slab64, s);
+            };
+            int done_b = 0;
+            GF_TRY(fused_loop(lat->dims, V, gauge, switch_precision, maxiter - done32, check32, kCoarse * check32,
+                              kStall32, false, false, sweep_grid(geo), fc, s, launch64, none, plain64, &done_b, out));
+            done32 += done_b;
+        }
     }
     out->sweeps_fp32 = done32;
     // switch: apply the accumulated transformation to the FP64 configuration,
@@ -1411,11 +1437,45 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     // that may meet it after a sweep or two: check it before the even update
     // from the start
     const bool near = switch_precision < kNearFactor * precision;
-    GF_TRY(fused_phase<double>(&tiles.lat, tiles.p, nullptr, gauge, omega, precision, left, 1, 0, 0, true, near, fc,
-                               s, after64, &done64, out));
-    out->sweeps = done32 + done64;
-    // back to the caller's links, projected (third rows, LandauGaugeFixingSU3_4D.cu:271)
-    HIP_TRY(launch_convert(GF_F64, kTiled12, tiles.p, GF_F64, lat->layout, U, V, 1, s));
+    if (gauge64) {
+        // FP64 sweeps of a second gauge transformation g64 (from 1) on the
+        // links G U0 G^dagger, read only: the same OR updates as sweeping the
+        // links, with 1056 instead of 1536 bytes per site-update; g64 is
+        // re-orthonormalised in every update, so nothing is reprojected
+        double *g64 = sg64.as<double>();
+        HIP_TRY(launch_gauge_identity64(g64, V, s));
+        GaugeSweepArgs gd{};
+        gd.U0 = (const float *)tiles.p;
+        gd.g = (float *)g64;
+        gd.state = fc.st;
+        gd.geo = geo;
+        gd.param = omega;
+        gd.remap = 1;
+        gd.links64 = 1;
+        auto launch64 = [&](int parity, int theta, double *partial, int) {
+            gd.parity = parity;
+            gd.partial = partial;
+            return launch_gsweep(gauge, GF_OR, theta, true, gd, s);
+        };
+        const int slab64 = wavefront_slab(gd, gauge, true, s);
+        auto plain64 = [&](int) {
+            if (slab64 <= 0) return hipErrorNotSupported;
+            return launch_gsweep_wavefront(gauge, GF_OR, true, gd, slab64, s);
+        };
+        auto none = [](int) -> gf_status { return GF_OK; };
+        GF_TRY(fused_loop(lat->dims, V, gauge, precision, left, 1, 0, 0, true, near, sweep_grid(geo), fc, s, launch64,
+                          none, plain64, &done64, out));
+        out->sweeps = done32 + done64;
+        // U = G64 (G U0 G^dagger) G64^dagger into the caller's links, projected
+        // (third rows, LandauGaugeFixingSU3_4D.cu:271)
+        HIP_TRY(launch_gauge_apply64(tiles.p, g64, (double *)U, lat->layout, geo, 1, s));
+    } else {
+        GF_TRY(fused_phase<double>(&tiles.lat, tiles.p, nullptr, gauge, omega, precision, left, 1, 0, 0, true, near,
+                                   fc, s, after64, &done64, out));
+        out->sweeps = done32 + done64;
+        // back to the caller's links, projected (third rows, LandauGaugeFixingSU3_4D.cu:271)
+        HIP_TRY(launch_convert(GF_F64, kTiled12, tiles.p, GF_F64, lat->layout, U, V, 1, s));
+    }
     HIP_TRY(hipEventRecord(ev.b, s));
     float ms = 0, ms1 = 0;
     HIP_TRY(hipEventSynchronize(ev.b));

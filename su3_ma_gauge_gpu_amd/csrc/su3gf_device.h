@@ -127,6 +127,10 @@ template <typename R> struct LinkRef {
 // instead of twelve 512-byte pieces V * 8 bytes apart.
 constexpr int kTiled12 = -12;
 __host__ __device__ inline int64_t tiled12_reals(int64_t V) { return 12 * 64 * ((V + 63) / 64); }
+// The FP64 gauge-transformation sweep addresses one direction of such links,
+// and g, as one buffer of 32-bit byte offsets: 12 doubles per site in whole
+// tiles stay below 2^31 bytes up to 22,369,600 sites (68^4 fits).
+__host__ __device__ inline bool gauge_sweep64_fits(int64_t V) { return 8 * tiled12_reals(V) < (int64_t(1) << 31); }
 
 template <typename R> __device__ __forceinline__ LinkRef<R> link_ref(R *base, int64_t V, int stride, int64_t idx, int mu)
 {

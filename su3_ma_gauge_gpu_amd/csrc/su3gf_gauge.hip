@@ -190,19 +190,82 @@ template <typename A> __device__ __forceinline__ void load_rows01_q24(const Fiel
         for (int j = 0; j < 3; j++) u.e[i][j] = cx(v[2 * (j + 3 * i)], v[2 * (j + 3 * i) + 1]);
 }
 
-// U0 rows 0,1 of the site at byte offset 4 i: FP32 planes (LF = 0) or 24-bit
-// fixed point in tiles (LF = 1)
+// FP64 storage (LF = 2, the FP64 phase of the mixed driver): g and each
+// direction of the links as 12 doubles per site in tiles of 64 sites
+// (kTiled12's layout): component c of site i at double
+// (i & ~63) * 12 + 64 c + (i & 63).  Byte offsets are 8 i, the component
+// stride of a tile 512 B.
+__device__ __forceinline__ Field make_tiled_field64(const double *base, int64_t V)
+{
+    Field f;
+    f.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)(8 * tiled12_reals(V)),
+                                            0x00020000);
+    f.V4 = 512u;
+    return f;
+}
+__device__ __forceinline__ uint32_t tile_off64(uint32_t boff) { return (boff & ~511u) * 12 + (boff & 511u); }
+
+template <typename A> __device__ __forceinline__ void load_rows01_64(const Field &f, uint32_t boff, M3<A> &u)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int comp = 2 * (j + 3 * i);
+            const double re = __builtin_bit_cast(
+                double, __builtin_amdgcn_raw_buffer_load_b64(f.r, (int)boff, (int)(comp * f.V4), 0));
+            const double im = __builtin_bit_cast(
+                double, __builtin_amdgcn_raw_buffer_load_b64(f.r, (int)boff, (int)((comp + 1) * f.V4), 0));
+            u.e[i][j] = cx((A)re, (A)im);
+        }
+}
+
+// byte offset of site i in a field of the storage of link form LF
+template <int LF> __device__ __forceinline__ uint32_t site_boff(int64_t i) { return (uint32_t)i * (LF == 2 ? 8u : 4u); }
+
+// U0 rows 0,1 of the site at byte offset boff (site_boff): FP32 planes (LF =
+// 0), 24-bit fixed point in tiles (LF = 1) or FP64 in tiles (LF = 2)
 template <typename A, int LF> __device__ __forceinline__ void load_links(const Field &f, uint32_t boff, M3<A> &u)
 {
-    if (LF) load_rows01_q24(f, tile_off(boff, 9), u);
+    if (LF == 2) load_rows01_64(f, tile_off64(boff), u);
+    else if (LF == 1) load_rows01_q24(f, tile_off(boff, 9), u);
     else load_rows01(f, boff, u);
 }
 
 // the field of U0_mu (one direction)
 template <int LF> __device__ __forceinline__ Field links_field(const float *U0, int64_t V, int mu)
 {
-    if (LF) return make_tiled_field(U0 + 9 * 64 * ((V + 63) / 64) * mu, V, 9);
+    if (LF == 2) return make_tiled_field64((const double *)U0 + tiled12_reals(V) * mu, V);
+    if (LF == 1) return make_tiled_field(U0 + 9 * 64 * ((V + 63) / 64) * mu, V, 9);
     return make_field(U0 + 12 * V * mu, V);
+}
+
+// g: FP32 tiles, or FP64 tiles with FP64 links (LF = 2)
+template <int LF> __device__ __forceinline__ Field g_field(const GaugeSweepArgs &a)
+{
+    if (LF == 2) return make_tiled_field64((const double *)a.g, a.geo.V);
+    return make_tiled_field(a.g, a.geo.V, 12);
+}
+template <typename A, int LF> __device__ __forceinline__ void load_g(const Field &f, uint32_t boff, M3<A> &u)
+{
+    if (LF == 2) load_rows01_64(f, tile_off64(boff), u);
+    else load_rows01(f, tile_off(boff, 12), u);
+}
+
+typedef unsigned int B64 __attribute__((__vector_size__(8)));  // the 64-bit buffer store's operand
+
+template <typename A> __device__ __forceinline__ void store_rows01_64(const Field &f, uint32_t boff, const M3<A> &u)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int comp = 2 * (j + 3 * i);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, (double)u.e[i][j].x), f.r, (int)boff,
+                                                  (int)(comp * f.V4), 0);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, (double)u.e[i][j].y), f.r, (int)boff,
+                                                  (int)((comp + 1) * f.V4), 0);
+        }
 }
 
 template <typename A> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
@@ -217,6 +280,12 @@ template <typename A> __device__ __forceinline__ void store_rows01(const Field &
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)u.e[i][j].y), f.r, (int)boff,
                                                   (int)((comp + 1) * f.V4), 0);
         }
+}
+
+template <typename A, int LF> __device__ __forceinline__ void store_g(const Field &f, uint32_t boff, const M3<A> &u)
+{
+    if (LF == 2) store_rows01_64(f, tile_off64(boff), u);
+    else store_rows01(f, tile_off(boff, 12), u);
 }
 
 // Z = rows 0,1 of X Y^dagger (Y complete), then row 2 of the SU(3) product
@@ -440,14 +509,14 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
                                                 int par, uint32_t site_off, M3<A> &X, M3<A> &Z)
 {
     M3<A> T, W;
-    const uint32_t y_off = (uint32_t)forward(a.geo, c, par, mu) * 4u;
-    const uint32_t z_off = (uint32_t)backward(a.geo, c, par, mu) * 4u;
+    const uint32_t y_off = site_boff<LF>(forward(a.geo, c, par, mu));
+    const uint32_t z_off = site_boff<LF>(backward(a.geo, c, par, mu));
     const Field fu = links_field<LF>(a.U0, a.geo.V, mu);
     load_links<A, LF>(fu, site_off, W);
-    load_rows01(fg, tile_off(y_off, 12), T);
+    load_g<A, LF>(fg, y_off, T);
     third_row(T);
     prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
-    load_rows01(fg, tile_off(z_off, 12), T);
+    load_g<A, LF>(fg, z_off, T);
     load_links<A, LF>(fu, z_off, W);
     third_row(W);
     prod_ab(Z, T, W);  // g(z) U0_mu(z)
@@ -457,7 +526,7 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
 // reads it again for the theta of the updated state) and g(x) rows 0,1 in
 // Gx: the three subgroup updates of g(x), its store, and the fused-check
 // partial sums of the 64-site block blk.
-template <typename A, int G, int ALGO, int THETA, typename SumS>
+template <typename A, int G, int ALGO, int THETA, int LF, typename SumS>
 __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &fg, uint32_t site_off, bool active,
                                           int64_t blk, int lane, const SumS &sum_s, M3<A> Gx, Philox &rng)
 {
@@ -474,7 +543,16 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
         sum_s(S);
         mac_adag(M, S, Gx);
     }
-    if (THETA == 1 && active) theta_from_m(M, th, gf);  // the state before the update
+    if ((THETA == 1 || THETA == 3) && active) theta_from_m(M, th, gf);  // the state before the update
+    if constexpr (THETA == 3) {  // read-only pass: theta only
+        th = wave64_sum(th);
+        gf = wave64_sum(gf);
+        if (lane == 0) {
+            a.partial[2 * blk] = th;
+            a.partial[2 * blk + 1] = gf;
+        }
+        return;
+    }
     {   // subgroup (0,2): reads M00 M02 M20 M22; M00 M10 M02 M12 M22 are read later
         A q[4];
         quat<A, 0, 2>(M, q);
@@ -503,7 +581,7 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
     }
 
     project_rows01(Gx);  // keep g unitary at the precision it is stored in
-    if (active) store_rows01(fg, tile_off(site_off, 12), Gx);
+    if (active) store_g<A, LF>(fg, site_off, Gx);
 
     if (THETA == 2 && active) {  // the state after the update, with g(x) as stored
         third_row(Gx);
@@ -541,7 +619,9 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 {
     constexpr int kW = gsweep_waves<G>();
     __shared__ A lds[18 * kW * 64];
+    static_assert(LF != 2 || sizeof(A) == 8, "FP64 storage takes FP64 arithmetic");
     if (a.state && a.state->done) return;  // converged: uniform early exit
+    if (THETA == 3 && !a.state->near) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mu = wave + (4 - kW);  // one direction per wave
     const int lane = threadIdx.x & 63;
@@ -557,11 +637,11 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
     const int64_t site = (active ? p : 0) + (parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
-    const uint32_t site_off = (uint32_t)site * 4u;
-    const Field fg = make_tiled_field(a.g, a.geo.V, 12);
+    const uint32_t site_off = site_boff<LF>(site);
+    const Field fg = g_field<LF>(a);
 
     M3<A> Gx;
-    if (wave == 0) load_rows01(fg, tile_off(site_off, 12), Gx);
+    if (wave == 0) load_g<A, LF>(fg, site_off, Gx);
 
     Philox rng;
     if (ALGO == GF_SA || ALGO == GF_RANDOM) site_rng(a, p, parity, rng);
@@ -602,7 +682,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
                 S.e[i][j] = cx(v[0], v[1]);
             }
     };
-    site_tail<A, G, ALGO, THETA>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng);
+    site_tail<A, G, ALGO, THETA, LF>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng);
 }
 
 // MAG on the gauge transformation.  MAG's subgroup quaternion is quadratic in
@@ -623,7 +703,9 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     constexpr int kX = 4 * 3 * 64;  // one exchange: [direction 4][D, Im c, Re c][lane 64]
     __shared__ A lds[2 * kX + (THETA ? kThetaVals * 4 * 64 : 0)];
     A *const tlds = lds + 2 * kX;
+    static_assert(LF != 2 || sizeof(A) == 8, "FP64 storage takes FP64 arithmetic");
     if (a.state && a.state->done) return;  // converged: uniform early exit
+    if (THETA == 3 && !a.state->near) return;
     const int mu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     int parity = a.parity;
@@ -638,11 +720,11 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     const int64_t site = (active ? p : 0) + (parity ? a.geo.Vh : 0);
     int c[4], par;
     site_coords(a.geo, site, c, par);
-    const uint32_t site_off = (uint32_t)site * 4u;
-    const Field fg = make_tiled_field(a.g, a.geo.V, 12);
+    const uint32_t site_off = site_boff<LF>(site);
+    const Field fg = g_field<LF>(a);
 
     M3<A> Gx;
-    load_rows01(fg, tile_off(site_off, 12), Gx);
+    load_g<A, LF>(fg, site_off, Gx);
     third_row(Gx);
     Philox rng;  // the same stream in every wave
     if (ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) site_rng(a, p, parity, rng);
@@ -672,7 +754,17 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
             if (active) theta_site<GF_MAG>(s3, th, gf);
         }
     };
-    if (THETA == 1) theta_pass();
+    if (THETA == 1 || THETA == 3) theta_pass();
+    if constexpr (THETA == 3) {  // read-only pass: theta only
+        if (mu != 0) return;
+        th = wave64_sum(th);
+        gf = wave64_sum(gf);
+        if (lane == 0) {
+            a.partial[2 * blk] = th;
+            a.partial[2 * blk + 1] = gf;
+        }
+        return;
+    }
 
     int buf = 0;
     auto subgroup = [&](auto I_, auto J_) {
@@ -715,7 +807,7 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     if (THETA == 2) theta_pass();
     if (mu != 0) return;
     project_rows01(Gx);
-    if (active) store_rows01(fg, tile_off(site_off, 12), Gx);
+    if (active) store_g<A, LF>(fg, site_off, Gx);
     if (THETA != 0) {
         th = wave64_sum(th);
         gf = wave64_sum(gf);
@@ -741,24 +833,38 @@ template <typename A, int ALGO, int THETA, int LF> void launch_k_mag(const Gauge
 
 template <typename A, int LF> hipError_t launch_mag_lf(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
 {
-    if (algo == GF_OR) {
-        if (theta == 1) launch_k_mag<A, GF_OR, 1, LF>(a, s);
-        else if (theta == 2) launch_k_mag<A, GF_OR, 2, LF>(a, s);
-        else launch_k_mag<A, GF_OR, 0, LF>(a, s);
-    } else if (theta != 0) {
-        return hipErrorInvalidValue;
-    } else if (algo == GF_MICRO) {
-        launch_k_mag<A, GF_MICRO, 0, LF>(a, s);
-    } else if (algo == GF_SA) {
-        launch_k_mag<A, GF_SA, 0, LF>(a, s);
-    } else if (algo == GF_SR) {
-        launch_k_mag<A, GF_SR, 0, LF>(a, s);
-    } else if (algo == GF_RANDOM) {  // MAG's random step keeps q3 = 0 (GaugeFixingSubgroupStep.hxx:566-575)
-        launch_k_mag<A, GF_RANDOM, 0, LF>(a, s);
+    if constexpr (LF == 2) {  // the FP64 phase: OR with the check, or the read-only theta pass
+        if constexpr (sizeof(A) != 8) {
+            return hipErrorInvalidValue;
+        } else {
+            if (algo != GF_OR) return hipErrorInvalidValue;
+            if (theta == 1) launch_k_mag<A, GF_OR, 1, 2>(a, s);
+            else if (theta == 2) launch_k_mag<A, GF_OR, 2, 2>(a, s);
+            else if (theta == 3) launch_k_mag<A, GF_OR, 3, 2>(a, s);
+            else launch_k_mag<A, GF_OR, 0, 2>(a, s);
+            return hipGetLastError();
+        }
     } else {
-        return hipErrorInvalidValue;
+        if (theta == 3) return hipErrorInvalidValue;
+        if (algo == GF_OR) {
+            if (theta == 1) launch_k_mag<A, GF_OR, 1, LF>(a, s);
+            else if (theta == 2) launch_k_mag<A, GF_OR, 2, LF>(a, s);
+            else launch_k_mag<A, GF_OR, 0, LF>(a, s);
+        } else if (theta != 0) {
+            return hipErrorInvalidValue;
+        } else if (algo == GF_MICRO) {
+            launch_k_mag<A, GF_MICRO, 0, LF>(a, s);
+        } else if (algo == GF_SA) {
+            launch_k_mag<A, GF_SA, 0, LF>(a, s);
+        } else if (algo == GF_SR) {
+            launch_k_mag<A, GF_SR, 0, LF>(a, s);
+        } else if (algo == GF_RANDOM) {  // MAG's random step keeps q3 = 0 (GaugeFixingSubgroupStep.hxx:566-575)
+            launch_k_mag<A, GF_RANDOM, 0, LF>(a, s);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
-    return hipGetLastError();
 }
 
 template <typename A, int G, int ALGO, int THETA, int LF>
@@ -770,6 +876,18 @@ void launch_k(const GaugeSweepArgs &a, hipStream_t s)
 
 template <typename A, int G> hipError_t launch_g(int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
 {
+    if (a.links64) {  // FP64 links and g: the FP64 phase, OR with the check or the read-only theta pass
+        if constexpr (sizeof(A) == 8) {
+            if (algo != GF_OR) return hipErrorInvalidValue;
+            if (theta == 1) launch_k<A, G, GF_OR, 1, 2>(a, s);
+            else if (theta == 2) launch_k<A, G, GF_OR, 2, 2>(a, s);
+            else if (theta == 3) launch_k<A, G, GF_OR, 3, 2>(a, s);
+            else launch_k<A, G, GF_OR, 0, 2>(a, s);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
+    if (theta == 3) return hipErrorInvalidValue;
     if (a.links24) {  // fixed-point links: OR of the mixed driver, SA and microcanonical of the annealing
         if (algo == GF_OR) {
             if (theta == 1) launch_k<A, G, GF_OR, 1, 1>(a, s);
@@ -835,7 +953,10 @@ template <typename R, int SI> __global__ __launch_bounds__(256) void k_pack24(co
 
 template <typename A> hipError_t launch_a(int gauge, int algo, int theta, const GaugeSweepArgs &a, hipStream_t s)
 {
-    if (gauge == GF_MAG) return a.links24 ? launch_mag_lf<A, 1>(algo, theta, a, s) : launch_mag_lf<A, 0>(algo, theta, a, s);
+    if (gauge == GF_MAG) {
+        if (a.links64) return launch_mag_lf<A, 2>(algo, theta, a, s);
+        return a.links24 ? launch_mag_lf<A, 1>(algo, theta, a, s) : launch_mag_lf<A, 0>(algo, theta, a, s);
+    }
     if (gauge == GF_LANDAU) return launch_g<A, GF_LANDAU>(algo, theta, a, s);
     if (gauge == GF_COULOMB) return launch_g<A, GF_COULOMB>(algo, theta, a, s);
     if (gauge == GF_U1XU1) return launch_g<A, GF_U1XU1>(algo, theta, a, s);
@@ -891,6 +1012,8 @@ hipError_t launch_pack24(int ti, int si, const void *U, uint32_t *P, int64_t V, 
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
 {
     if (!gauge_sweep_fits(a.geo.V)) return hipErrorInvalidValue;
+    if (a.links64 && (!fp64_arith || !gauge_sweep64_fits(a.geo.V))) return hipErrorInvalidValue;
+    if (theta == 3 && !a.state) return hipErrorInvalidValue;
     GaugeSweepArgs b = a;
     const int64_t half_slice = a.geo.Vh / a.geo.N[0];  // sites of one parity in a timeslice
     if (b.remap && a.geo.N[0] % 8 == 0 && half_slice % 64 == 0) {
@@ -912,6 +1035,7 @@ hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const G
                                    hipStream_t s)
 {
     if (!gauge_sweep_fits(a.geo.V)) return hipErrorInvalidValue;
+    if (a.links64 && (!fp64_arith || !gauge_sweep64_fits(a.geo.V))) return hipErrorInvalidValue;
     const int N = a.geo.N[0];
     const int64_t grid = sweep_grid(a.geo);
     // blocks per timeslice: whole 64-site blocks, or a Coulomb lattice's

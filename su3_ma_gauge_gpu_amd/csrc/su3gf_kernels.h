@@ -89,6 +89,8 @@ struct GaugeSweepArgs {
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
     int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR, SA, microcanonical)
+    int links64;       // U0 and g in FP64 tiles (kTiled12, 12 doubles per site): the mixed driver's
+                       // FP64 phase (OR with FP64 arithmetic; theta 0-3, 3 the read-only pass)
     // Wavefront launch (wf = 1, set by launch_gsweep_wavefront): one launch
     // sweeps the even sites of logical blocks [wf_even0, wf_even0 + wf_even_n)
     // with its first wf_even_grid blocks and the odd sites of
@@ -102,8 +104,9 @@ struct GaugeSweepArgs {
     uint32_t counter_odd;
 };
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
-// 1 pre-update (even kernel), 2 post-update (odd kernel); fp64_arith:
-// compute in FP64 on the FP32 data
+// 1 pre-update (even kernel), 2 post-update (odd kernel), 3 the read-only
+// theta pass (links64 only, while state->near is set); fp64_arith: compute
+// in FP64 on the FP32 data (required with links64)
 hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
 // One whole sweep (even then odd sites, no fused check) of a 4D lattice as a
 // wavefront over timeslices: launch k updates the even sites of timeslices
@@ -161,6 +164,12 @@ hipError_t launch_project_gauge(float *g, int64_t V, hipStream_t s);
 // U0 FP64 in layout s0; out of type `to` in layout so (may alias U0 if same type/layout)
 hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *out, int to, int so, const Geom &geo,
                               hipStream_t s);
+// the FP64 phase's g (12 doubles per site in 64-site tiles) set to 1, and
+// out = G U1 G^dagger with U1 in kTiled12 (FP64), out FP64 in layout so
+// (18 or 12), reprojected when reun
+hipError_t launch_gauge_identity64(double *g, int64_t V, hipStream_t s);
+hipError_t launch_gauge_apply64(const double *U1, const double *g, double *out, int so, const Geom &geo, int reun,
+                                hipStream_t s);
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
                              double stall_below, int exact_end, double near_below, int near, hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep

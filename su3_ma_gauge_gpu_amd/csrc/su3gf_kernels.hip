@@ -227,6 +227,87 @@ hipError_t launch_gauge_apply(const double *U0, int s0, const float *g, void *ou
     return hipGetLastError();
 }
 
+// The FP64 phase of the mixed driver sweeps an FP64 g (12 doubles per site
+// in 64-site tiles, kTiled12's layout with one field) on read-only FP64 links.
+__global__ __launch_bounds__(kBlock) void k_gauge_identity64(double *g, int64_t V)
+{
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= V) return;
+    M3<double> u;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) u.e[i][j] = cx((double)(i == j), 0.0);
+    store12(link_ref(g, V, kTiled12, idx, 0), u);
+}
+
+hipError_t launch_gauge_identity64(double *g, int64_t V, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gauge_identity64, dim3(blocks_for(V)), dim3(kBlock), 0, s, g, V);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ void gauge64_at(const double *g, int64_t V, int64_t idx, M3<double> &G)
+{
+    load12(link_ref(const_cast<double *>(g), V, kTiled12, idx, 0), G);
+    project_e<false>(G);
+}
+
+// out_mu(x) = G(x) U1_mu(x) G(x+mu)^dagger, G the FP64 g (reprojected), U1
+// the FP64 links in kTiled12; out in layout SO (18 or 12), reprojected when
+// reun (the final projectSU3 of the run), one thread per link
+template <int SO>
+__global__ __launch_bounds__(kBlock) void k_gauge_apply64(const double *U1, const double *g, double *out, Geom geo,
+                                                          int reun)
+{
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 4 * geo.V) return;
+    const int mu = (int)(t / geo.V);
+    const int64_t idx = t - (int64_t)mu * geo.V;
+    int c[4], par;
+    site_coords(geo, idx, c, par);
+    const int64_t fw = forward(geo, c, par, mu);
+    M3<double> Gx, Gf, U;
+    gauge64_at(g, geo.V, idx, Gx);
+    gauge64_at(g, geo.V, fw, Gf);
+    load12(link_ref(const_cast<double *>(U1), geo.V, kTiled12, idx, mu), U);
+    reconstruct_e<false>(U);
+    M3<double> T, M;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            Cx<double> s = Gx.e[i][0] * U.e[0][j];
+            s = s + Gx.e[i][1] * U.e[1][j];
+            s = s + Gx.e[i][2] * U.e[2][j];
+            T.e[i][j] = s;
+        }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            Cx<double> s = mul_conj(T.e[i][0], Gf.e[j][0]);
+            s = s + mul_conj(T.e[i][1], Gf.e[j][1]);
+            s = s + mul_conj(T.e[i][2], Gf.e[j][2]);
+            M.e[i][j] = s;
+        }
+    if (reun) project_e<false>(M);
+    else reconstruct_e<false>(M);
+    const LinkRef<double> lo = link_ref(out, geo.V, SO, idx, mu);
+    store12(lo, M);
+    if (SO == 18) store_row2(lo, M);
+}
+
+hipError_t launch_gauge_apply64(const double *U1, const double *g, double *out, int so, const Geom &geo, int reun,
+                                hipStream_t s)
+{
+    const dim3 grid(blocks_for(4 * geo.V)), block(kBlock);
+    if (so == 18) hipLaunchKernelGGL((k_gauge_apply64<18>), grid, block, 0, s, U1, g, out, geo, reun);
+    else if (so == 12) hipLaunchKernelGGL((k_gauge_apply64<12>), grid, block, 0, s, U1, g, out, geo, reun);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ driver state
 __global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
                              double stall_below, int exact_end, double near_below, int near)

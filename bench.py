@@ -44,7 +44,7 @@ import su3_ma_gauge_gpu_amd as gf  # noqa: E402
 from su3_ma_gauge_gpu_amd import _lib  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-MODE_NAMES = {0: "links_f32+g", 1: "gauge_f32", 2: "gauge_f32_f64arith", 3: "gauge_q24"}
+MODE_NAMES = {0: "links_f32+g", 1: "gauge_f32", 2: "gauge_f32_f64arith", 3: "gauge_q24", 4: "fp64_gauge_sweep"}
 
 
 def parse():
@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--mag-precision", type=float, default=1e-12)
     ap.add_argument("--maxiter-mag", type=int, default=600000)
     ap.add_argument("--u1xu1-sweeps", type=int, default=1000)
+    ap.add_argument("--mag-runs", default="both", choices=["both", "mixed", "fp64"],
+                    help="--workload mag: the mixed and the all-FP64 run, or one of them (each takes minutes at 48^4)")
     ap.add_argument("--workload", default="landau",
                     choices=["landau", "coulomb", "gribov", "mag"],
                     help="landau: the 32^4 headline run (default); coulomb: 32^3x64 all slices at once; "
@@ -245,20 +247,26 @@ def workload_mag(args):
     against all-FP64 to theta < --mag-precision, then U1xU1 OR sweeps on the
     MAG result; MAG's sweep kernels timed one by one.  At this size OR needs
     a few hundred thousand sweeps: expect minutes per run."""
-    line = mag_line(args, 48, args.maxiter_mag)
-    line.update(mag_kernels(args, 48))
+    line = mag_line(args, 48, args.maxiter_mag, args.mag_runs)
+    if args.mag_runs == "both":
+        line.update(mag_kernels(args, 48))
     print(json.dumps(line), flush=True)
 
 
-def _run_chunked(lat, work, gauge, args, prec, maxiter, stream, what):
-    """gf_run_fast to theta < prec in chunks of 20000 sweeps (a progress line
-    after each): one run, as the sweeps do not depend on where a chunk ends
-    but for the reprojection at each chunk's first sweep.  Returns (sweeps,
-    ms, last result)."""
+def _run_chunked(lat, work, gauge, args, prec, maxiter, stream, what, gauge_sweep=False):
+    """FP64 OR to theta < prec in chunks of 20000 sweeps (a progress line
+    after each): gf_run_fast (link sweeps), or with gauge_sweep the FP64
+    gauge-transformation sweep (gf_run_mixed without its FP32 phase).  One
+    run, as the sweeps do not depend on where a chunk ends but for a
+    reprojection at each chunk boundary.  Returns (sweeps, ms, last result)."""
     sweeps, ms, r = 0, 0.0, None
     while sweeps < maxiter:
-        r = gf.run_fast(lat, work, gauge, args.omega, prec, min(20000, maxiter - sweeps), args.reproject,
-                        stream=stream)
+        n = min(20000, maxiter - sweeps)
+        if gauge_sweep:
+            r = gf.run_mixed(lat, work, gauge, args.omega, prec, prec, n, mode=gf.MIXED_FP64,
+                             reproject=args.reproject, stream=stream)
+        else:
+            r = gf.run_fast(lat, work, gauge, args.omega, prec, n, args.reproject, stream=stream)
         sweeps += r.sweeps
         ms += r.ms_total
         print(f"bench: {what}: {sweeps} sweeps, theta {r.theta:.3e}, {ms / 1e3:.1f} s", file=sys.stderr, flush=True)
@@ -277,11 +285,12 @@ def mag_quality(lat, U, stream):
     return g, t
 
 
-def mag_line(args, L: int, maxiter: int) -> dict:
+def mag_line(args, L: int, maxiter: int, runs: str = "both") -> dict:
     """MAG of an L^4 lattice (sigma 0.35, seed 1, Haar seed 2) to theta <
     --mag-precision: mixed (the FP32 phase in --mode until theta < 1e-6, then
-    FP64 sweeps) against all-FP64; then --u1xu1-sweeps U1xU1 OR sweeps on
-    each MAG result, which leave MAG's functional and theta unchanged (a
+    FP64 sweeps) against all-FP64 (runs: "both", or one of "mixed", "fp64",
+    for runs too long to make together); then --u1xu1-sweeps U1xU1 OR sweeps
+    on each MAG result, which leave MAG's functional and theta unchanged (a
     diagonal gauge rotation changes neither |U_jj| nor |X_k|)."""
     dims = (L, L, L, L)
     V = int(np.prod(dims))
@@ -295,24 +304,36 @@ def mag_line(args, L: int, maxiter: int) -> dict:
     def reset():
         _lib.call("gf_memcpy_async", C.c_void_p(work.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
 
-    # mixed: the FP32 phase to 1e-6 (gf_run_mixed with precision 1e-6 ends on
-    # its first FP64 check), then FP64 sweeps to the precision
-    reset()
-    m32 = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, 1e-6, maxiter, mode=args.mode, refresh=args.refresh,
-                       check32=args.check32, reproject=args.reproject, stream=stream)
-    print(f"bench: mag {L}^4 mixed, FP32 phase: {m32}", file=sys.stderr, flush=True)
-    n64, ms64, m = _run_chunked(lat, work, gf.MAG, args, prec, maxiter - m32.sweeps, stream, f"mag {L}^4 mixed, FP64")
-    mixed = {"time_to_theta_s": (m32.ms_total + ms64) / 1e3, "sweeps": m32.sweeps + n64,
-             "sweeps_fp32": m32.sweeps_fp32, "time_fp32_s": m32.ms_fp32 / 1e3, "converged": m.converged,
-             "gff": m.gff, "theta": m.theta}
-    mixed_fixed = gf.DeviceBuffer(lat.nbytes)
-    _lib.call("gf_memcpy_async", C.c_void_p(mixed_fixed.ptr), C.c_void_p(work.ptr), C.c_size_t(lat.nbytes), stream)
-    reset()
-    nf, msf, f = _run_chunked(lat, work, gf.MAG, args, prec, maxiter, stream, f"mag {L}^4 FP64")
-    fp64 = {"time_to_theta_s": msf / 1e3, "sweeps": nf, "converged": f.converged, "gff": f.gff, "theta": f.theta}
+    mixed = fp64 = m = f = None
+    fixed = []  # (name, buffer holding that run's MAG result)
+    if runs in ("both", "mixed"):
+        # the FP32 phase to 1e-6 (gf_run_mixed with precision 1e-6 ends on its
+        # first FP64 check), then FP64 sweeps to the precision as the mixed
+        # driver's FP64 phase runs them (the FP64 gauge-transformation sweep
+        # in the gauge modes), in chunks
+        reset()
+        m32 = gf.run_mixed(lat, work, gf.MAG, args.omega, 1e-6, 1e-6, maxiter, mode=args.mode, refresh=args.refresh,
+                           check32=args.check32, reproject=args.reproject, stream=stream)
+        print(f"bench: mag {L}^4 mixed, FP32 phase: {m32}", file=sys.stderr, flush=True)
+        n64, ms64, m = _run_chunked(lat, work, gf.MAG, args, prec, maxiter - m32.sweeps, stream,
+                                    f"mag {L}^4 mixed, FP64", gauge_sweep=args.mode != 0)
+        mixed = {"time_to_theta_s": (m32.ms_total + ms64) / 1e3, "sweeps": m32.sweeps + n64,
+                 "sweeps_fp32": m32.sweeps_fp32, "time_fp32_s": m32.ms_fp32 / 1e3, "converged": m.converged,
+                 "gff": m.gff, "theta": m.theta}
+        if runs == "both":
+            keep = gf.DeviceBuffer(lat.nbytes)
+            _lib.call("gf_memcpy_async", C.c_void_p(keep.ptr), C.c_void_p(work.ptr), C.c_size_t(lat.nbytes), stream)
+            fixed.append(("mixed", keep))
+        else:
+            fixed.append(("mixed", work))
+    if runs in ("both", "fp64"):
+        reset()
+        nf, msf, f = _run_chunked(lat, work, gf.MAG, args, prec, maxiter, stream, f"mag {L}^4 FP64")
+        fp64 = {"time_to_theta_s": msf / 1e3, "sweeps": nf, "converged": f.converged, "gff": f.gff, "theta": f.theta}
+        fixed.insert(0, ("fp64", work))
     # U1xU1 on each MAG result
     u1 = {}
-    for name, buf in (("fp64", work), ("mixed", mixed_fixed)):
+    for name, buf in fixed:
         g0, t0 = mag_quality(lat, buf, stream)
         if name == "fp64":
             u = gf.run_fast(lat, buf, gf.U1XU1, args.omega, 1e-30, args.u1xu1_sweeps, args.reproject, stream=stream)
@@ -323,23 +344,24 @@ def mag_line(args, L: int, maxiter: int) -> dict:
         u1[name] = {"sweeps": u.sweeps, "time_s": u.ms_total / 1e3, "u1xu1_gff": u.gff,
                     "mag_gff_rel_change": abs(g1 - g0) / g0, "mag_theta_before": t0, "mag_theta_after": t1}
         print(f"bench: u1xu1 on the {name} MAG result ({L}^4): {u}", file=sys.stderr, flush=True)
-    del mixed_fixed
+    del fixed
+    main_run = mixed if mixed is not None else fp64
     line = _base_line(f"MAG OR time to theta<{prec:g} at {L}^4: mixed vs FP64, then U1xU1",
-                      mixed["sweeps"] * V / mixed["time_to_theta_s"], "site-updates/s", mixed["time_to_theta_s"] * 1e3,
-                      1, 0,
+                      main_run["sweeps"] * V / main_run["time_to_theta_s"], "site-updates/s",
+                      main_run["time_to_theta_s"] * 1e3, 1, 0,
                       {"workload": f"mag-or-{L}x{L}x{L}x{L}-mixed+u1xu1", "dims": list(dims), "sigma": 0.35,
                        "seed": 1, "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
-                       "mode": MODE_NAMES[args.mode], "u1xu1_sweeps": args.u1xu1_sweeps})
-    both = mixed["converged"] and fp64["converged"]
-    line.update({
-        "mixed": mixed, "fp64": fp64,
-        "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
-        # a time-to-theta ratio only when both runs reached the precision
-        "speedup_mixed_vs_fp64": fp64["time_to_theta_s"] / mixed["time_to_theta_s"] if both else None,
-        "u1xu1": u1,
-        "u1xu1_gff_rel_diff_mixed_vs_fp64": abs(u1["mixed"]["u1xu1_gff"] - u1["fp64"]["u1xu1_gff"])
-        / abs(u1["fp64"]["u1xu1_gff"]),
-    })
+                       "mode": MODE_NAMES[args.mode], "u1xu1_sweeps": args.u1xu1_sweeps, "runs": runs})
+    line.update({"mixed": mixed, "fp64": fp64, "u1xu1": u1})
+    if mixed is not None and fp64 is not None:
+        both = mixed["converged"] and fp64["converged"]
+        line.update({
+            "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
+            # a time-to-theta ratio only when both runs reached the precision
+            "speedup_mixed_vs_fp64": fp64["time_to_theta_s"] / mixed["time_to_theta_s"] if both else None,
+            "u1xu1_gff_rel_diff_mixed_vs_fp64": abs(u1["mixed"]["u1xu1_gff"] - u1["fp64"]["u1xu1_gff"])
+            / abs(u1["fp64"]["u1xu1_gff"]),
+        })
     return line
 
 
@@ -357,13 +379,13 @@ def mag_kernels(args, L: int) -> dict:
     U32 = gf.DeviceBuffer(lat32.nbytes)
     kms, frac = {}, {}
     for var, name, bsu in ((4, "gauge_sweep_q24", 432), (2, "gauge_sweep_f32", 528), (1, "sweep_f32_r12_gauge", 864),
-                           (0, "sweep_f64_tiled12", 1536)):
-        if var:
+                           (0, "sweep_f64_tiled12", 1536), (7, "gauge_sweep_f64_tiled12", 1056)):
+        if var in (1, 2, 4):
             _lib.call("gf_convert", C.byref(lat.struct), C.c_void_p(src.ptr), C.byref(lat32.struct),
                       C.c_void_p(U32.ptr), 0, stream)
             ms = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var, gf.MAG)
         else:
-            ms = kernel_time_ms(lat, src, stream, args.kernel_sweeps, 0, gf.MAG)
+            ms = kernel_time_ms(lat, src, stream, args.kernel_sweeps, var, gf.MAG)
         kms[name] = ms
         frac[name] = (V / 2) * bsu / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     return {f"kernel_ms_{L}": kms, f"kernel_hbm_frac_{L}": frac}
@@ -469,18 +491,24 @@ def landau48_line(args) -> dict:
     reset()
     f = gf.run_fast(lat64, work, gf.LANDAU, args.omega, args.precision, args.maxiter, args.reproject, stream=stream)
     print(f"bench: landau 48^4 fp64 {f}", file=sys.stderr, flush=True)
+    reset()
+    fg = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
+                      mode=gf.MIXED_FP64, stream=stream)
+    print(f"bench: landau 48^4 fp64 gauge sweep {fg}", file=sys.stderr, flush=True)
     U32 = gf.DeviceBuffer(lat32.nbytes)
     _lib.call("gf_convert", C.byref(lat64.struct), C.c_void_p(src.ptr), C.byref(lat32.struct), C.c_void_p(U32.ptr),
               0, stream)
     kms = {"gauge_sweep_q24": kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, 4),
            "gauge_sweep_q24_wavefront": kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, 6),
            "sweep_f64_tiled12": kernel_time_ms(lat64, src, stream, args.kernel_sweeps, 0),
+           "gauge_sweep_f64_tiled12": kernel_time_ms(lat64, src, stream, args.kernel_sweeps, 7),
            "dropin_quality_f64_pattern18": kernel_time_ms(lat64, src, stream, args.kernel_sweeps, 5)}
     # per half-sweep (V/2 site-updates); the wavefront's figure is half a
     # whole sweep's time, so its fraction counts the algorithmic bytes of the
     # half-sweeps it replaces
     kbytes = {"gauge_sweep_q24": (V // 2) * 432, "gauge_sweep_q24_wavefront": (V // 2) * 432,
-              "sweep_f64_tiled12": (V // 2) * 16 * 12 * 8, "dropin_quality_f64_pattern18": V * 4 * 12 * 8}
+              "sweep_f64_tiled12": (V // 2) * 16 * 12 * 8, "gauge_sweep_f64_tiled12": (V // 2) * 1056,
+              "dropin_quality_f64_pattern18": V * 4 * 12 * 8}
     both = m.converged == 1 and f.converged == 1
     return {
         "config": {"workload": "landau-or-48x48x48x48-mixed", "dims": list(dims), "sigma": args.sigma,
@@ -492,6 +520,9 @@ def landau48_line(args) -> dict:
         "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
         "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
         "speedup_mixed_vs_fp64": f.ms_total / m.ms_total if both else None,
+        "fp64_gauge_sweep": {"time_to_theta_s": fg.ms_total / 1e3, "sweeps": fg.sweeps, "converged": fg.converged,
+                             "gff_rel_diff_vs_fp64": abs(fg.gff - f.gff) / abs(f.gff),
+                             "speedup_vs_fp64_link_sweeps": f.ms_total / fg.ms_total if fg.converged else None},
         "site_updates_per_s_mixed": m.sweeps * V / (m.ms_total / 1e3),
         "kernel_ms": kms,
         "kernel_hbm_frac": {k: kbytes[k] / (kms[k] / 1e3) / 1e9 / HBM_PEAK_GBS for k in kms},
@@ -578,6 +609,12 @@ def main():
         fp64 = gf.run_fast(lat64, work, gf.LANDAU, args.omega, args.precision, args.maxiter, args.reproject,
                            stream=stream)
         print(f"bench: fp64 {fp64}", file=sys.stderr, flush=True)
+    # all-FP64 on the FP64 gauge-transformation sweep (gf_run_mixed without
+    # its FP32 phase): fewer bytes per site-update than the link sweep
+    fp64g = None
+    if not args.no_fp64:
+        fp64g = mixed(gf.MIXED_FP64, args.switch)
+        print(f"bench: fp64 gauge sweep {fp64g}", file=sys.stderr, flush=True)
 
     # the drop-in path end to end: the reference app's OR loop (gf_run_or:
     # reference-order kernels, the quality kernel and its reduction with a
@@ -616,8 +653,11 @@ def main():
         fresh32()
         kms[name] = kernel_time_ms(lat32, U32, stream, args.kernel_sweeps, var)
     reset()
-    # the FP64 phase's kernel, on the drivers' tiled copy of the links
+    # the FP64 kernels on the drivers' tiled copy of the links: the link sweep
+    # (gf_run_fast) and the gauge-transformation sweep (the FP64 phase of the
+    # gauge modes)
     kms["sweep_f64_tiled12"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 0)
+    kms["gauge_sweep_f64_tiled12"] = kernel_time_ms(lat64, work, stream, args.kernel_sweeps, 7)
     reset()
     kms["dropin_sweep_f64_pattern18"] = dropin_kernel_ms(lat64, work, stream, args.kernel_sweeps)
     # the drop-in quality kernel (GaugeFixingStats::generateGaugeQuality's per-site pass)
@@ -627,13 +667,14 @@ def main():
     #   FP32 links + g: also the site's 12-float g read and written (+96 B);
     #   gauge-transformation sweep: U0 read once (8 x 48 B), g of the other parity read once
     #   (48 B), the site's g read and written (96 B): 528 B; with U0 in 24-bit fixed
-    #   point (8 x 36 B): 432 B
+    #   point (8 x 36 B): 432 B; all in FP64 (links 8 x 96 B, g 96 + 192 B): 1056 B
     kbytes = {
         "sweep_f32_r12_gauge": (V // 2) * (16 * 12 * 4 + 2 * 12 * 4),
         "gauge_sweep_f32": (V // 2) * 528,
         "gauge_sweep_f64arith": (V // 2) * 528,
         "gauge_sweep_q24": (V // 2) * 432,
         "sweep_f64_tiled12": (V // 2) * 16 * 12 * 8,
+        "gauge_sweep_f64_tiled12": (V // 2) * 1056,
         "dropin_sweep_f64_pattern18": (V // 2) * 16 * 12 * 8,
         # quality: every link's rows 0,1 read once per launch (V sites x 4 links x 12 reals)
         "dropin_quality_f64_pattern18": V * 4 * 12 * 8,
@@ -642,8 +683,9 @@ def main():
     t64 = (r.ms_total - r.ms_fp32) / max(r.sweeps - r.sweeps_fp32, 1)
     k32name = {0: "sweep_f32_r12_gauge", 1: "gauge_sweep_f32", 2: "gauge_sweep_f64arith",
                3: "gauge_sweep_q24"}[args.mode]
-    dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms["sweep_f64_tiled12"]
-    kname = k32name if dom32 else "sweep_f64_tiled12"
+    k64name = "sweep_f64_tiled12" if args.mode == 0 else "gauge_sweep_f64_tiled12"
+    dom32 = r.sweeps_fp32 * kms[k32name] >= (r.sweeps - r.sweeps_fp32) * kms[k64name]
+    kname = k32name if dom32 else k64name
     achieved = kbytes[kname] / (kms[kname] / 1e3) / 1e9
     traffic = None
     tfile = ROOT / "profiles" / "traffic.json"
@@ -696,6 +738,10 @@ def main():
         "gff_fp64": None if fp64 is None else fp64.gff,
         "gff_rel_diff_mixed_vs_fp64": None if fp64 is None else abs(r.gff - fp64.gff) / abs(fp64.gff),
         "speedup_mixed_vs_fp64": None if fp64 is None else fp64.ms_total / r.ms_total,
+        "fp64_gauge_sweep": None if fp64g is None else {
+            "time_to_theta_s": fp64g.ms_total / 1e3, "sweeps": fp64g.sweeps, "gff": fp64g.gff,
+            "theta": fp64g.theta, "gff_rel_diff_vs_fp64": abs(fp64g.gff - fp64.gff) / abs(fp64.gff),
+            "speedup_vs_fp64_link_sweeps": fp64.ms_total / fp64g.ms_total},
         "mixed_switch_1e-6": None if ns is None else {
             "time_to_theta_s": ns.ms_total / 1e3, "sweeps": ns.sweeps, "sweeps_fp32": ns.sweeps_fp32,
             "gff": ns.gff, "speedup_vs_fp64": fp64.ms_total / ns.ms_total},

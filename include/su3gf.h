@@ -63,7 +63,8 @@ typedef enum {
     GF_MIXED_LINKS = 0,   /* FP32 links updated together with an FP32 g(x) (any gauge) */
     GF_MIXED_GAUGE32 = 1, /* g(x) swept on read-only FP32 links, FP32 arithmetic */
     GF_MIXED_GAUGE64 = 2, /* g(x) swept on read-only FP32 links, FP64 arithmetic */
-    GF_MIXED_GAUGE24 = 3  /* g(x) swept on read-only links in 24-bit fixed point, FP32 arithmetic */
+    GF_MIXED_GAUGE24 = 3, /* g(x) swept on read-only links in 24-bit fixed point, FP32 arithmetic */
+    GF_MIXED_FP64 = 4     /* no FP32 phase: the FP64 phase from the start */
 } gf_mixed_mode;
 
 typedef struct {
@@ -290,7 +291,9 @@ gf_status gf_run_fast_noalloc(const gf_lattice *lat, void *U, int gauge, float o
  * pass, nsweeps launches, any dtype and layout; the time is per launch);
  * 6 as 4, each sweep run as the mixed run's FP32 phase runs a sweep without
  * a check (a wavefront over timeslices where gf_set_wavefront's choice is
- * one), the time per half of a sweep.
+ * one), the time per half of a sweep; 7 (lat FP64) the FP64
+ * gauge-transformation sweep with the fused check, on the tiled copy of the
+ * links held read only (the FP64 phase of gf_run_mixed's gauge modes).
  * U is modified by variants 0 and 1.  Any gauge but Coulomb on a single
  * timeslice. */
 gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps, int variant,
@@ -320,12 +323,17 @@ gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *s
  * at the rounding floor of single precision: 8 checks in a row -- 25 for
  * MAG -- with theta below 1e-9 -- 1e-5 for MAG -- that do not lower its best
  * theta by 10%); then
- * U <- G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
+ * U1 = G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
  * continue to theta < precision, checked every sweep; the array then holds
  * the state that met it, as in gf_run_fast (with switch_precision <
  * 2 * precision the FP64 checks are made before the even update from the
- * first sweep on).
+ * first sweep on).  In the gauge-transformation modes (and GF_MIXED_FP64)
+ * the FP64 phase sweeps a second gauge transformation g64 (FP64, from 1) on
+ * U1 held read only, the same OR updates with 1056 instead of 1536 bytes per
+ * site-update, and writes g64 U1 g64^dagger back, projected; it takes lattices
+ * up to 22,369,600 sites, 68^4 (the link mode, and larger lattices, sweep the FP64
+ * links themselves, reprojected every `reproject` sweeps).
  *   mode GF_MIXED_LINKS: a compact FP32 copy of the links is swept and g
  *     updated alongside; every `refresh` sweeps the FP32 links are rebuilt
  *     as G U0 G^dagger (reprojected every `reproject` sweeps in between).
@@ -343,6 +351,9 @@ gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *s
  *   switch_precision continues on the same g and links with FP64
  *   arithmetic (the GAUGE64 kernel, whose floor is lower) until
  *   switch_precision or stagnation again; out->sweeps_fp32 counts both.
+ *   mode GF_MIXED_FP64: no FP32 phase (switch_precision, refresh and check32
+ *     unused): the FP64 phase alone, on the caller's links -- an all-FP64
+ *     run, on the gauge-transformation sweep where the volume allows.
  * The gauge-transformation modes take at most 44,739,200 sites (80^4 fits;
  * GF_ERR_BAD_SHAPE beyond), GF_MIXED_LINKS any volume below 2^31.
  * No reference counterpart (the reference's DPUPDATES is a compile-time

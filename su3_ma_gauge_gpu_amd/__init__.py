@@ -37,6 +37,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     MIXED_GAUGE32,
     MIXED_GAUGE64,
     MIXED_GAUGE24,
+    MIXED_FP64,
     MIXED_LINKS,
     OR,
     PATTERN18,
@@ -509,7 +510,8 @@ def run_mixed(lattice: Lattice, U, gauge: int = LANDAU, omega: float = 1.7, swit
     ``switch_precision`` (checked every ``check32`` sweeps), the transformation
     applied to the FP64 configuration, then FP64 fused-check sweeps to
     ``precision``.  ``mode``: MIXED_GAUGE24 (default), MIXED_GAUGE32,
-    MIXED_GAUGE64 or MIXED_LINKS -- see gf_run_mixed in include/su3gf.h."""
+    MIXED_GAUGE64, MIXED_LINKS, or MIXED_FP64 (no FP32 phase: the FP64 phase
+    alone) -- see gf_run_mixed in include/su3gf.h."""
     if mode is None:
         mode = MIXED_GAUGE24
     r = _lib.gf_run_result()

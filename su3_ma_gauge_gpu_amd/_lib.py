@@ -17,7 +17,7 @@ UPDATE_DP = 0x100  # OR'ed into an update rule: computed in double on FP32 links
 SA_GRAPH = 0x10  # OR'ed into an annealing mode: the schedule captured into one HIP graph
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
-MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24 = 0, 1, 2, 3
+MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24, MIXED_FP64 = 0, 1, 2, 3, 4
 CRIT_MAX, CRIT_AVERAGE = 0, 1
 FILE_PLAIN, FILE_HEADERONLY, FILE_VOGT, FILE_ILDG, FILE_QCDSTAG = range(5)
 REAL_STANDARD, REAL_DOUBLE, REAL_FLOAT = range(3)

@@ -1186,12 +1186,15 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U || !ms_per_kernel || nsweeps <= 0) return fail(GF_ERR_BAD_ARG, "bad argument");
-    if (variant < 0 || variant > 6) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
+    if (variant < 0 || variant > 7) return fail(GF_ERR_BAD_ARG, "unknown sweep variant %d", variant);
     if (gauge == GF_COULOMB && lat->dims[0] == 1) return fail(GF_ERR_BAD_ARG, "gf_time_sweeps times 4D lattices");
     if (lat->dtype == GF_F32 && lat->layout != GF_LAYOUT_R12)
         return fail(GF_ERR_BAD_ARG, "FP32 fused sweeps work on the compact 12-real layout");
-    if (variant > 0 && variant != 5 && lat->dtype != GF_F32)
+    if (variant > 0 && variant != 5 && variant != 7 && lat->dtype != GF_F32)
         return fail(GF_ERR_BAD_ARG, "variants 1-4 and 6 time FP32 kernels");
+    if (variant == 7 && lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "variant 7 times an FP64 kernel");
+    if (variant == 7 && !gauge_sweep64_fits(V))
+        return fail(GF_ERR_BAD_SHAPE, "the FP64 gauge-transformation sweep takes at most 22,369,600 sites");
     if (variant >= 2 && variant != 5) GF_TRY(check_gauge_sweep_volume(V));
     hipStream_t s = S(stream);
     if (variant == 5) {
@@ -1225,7 +1228,11 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     struct {
         float *p = nullptr;
     } gb;
-    if (variant > 0) {
+    if (variant == 7) {  // an FP64 g on the tiles, as the FP64 phase of gf_run_mixed's gauge modes
+        HIP_TRY(gbuf.alloc(sizeof(double) * gauge_floats(V), s));
+        gb.p = gbuf.as<float>();
+        HIP_TRY(launch_gauge_identity64(gbuf.as<double>(), V, s));
+    } else if (variant > 0) {
         HIP_TRY(gbuf.alloc(sizeof(float) * gauge_floats(V), s));
         gb.p = gbuf.as<float>();
         HIP_TRY(launch_gauge_identity(gb.p, V, s));
@@ -1263,7 +1270,12 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
             }
             for (int par = 0; par < 2; par++) {
                 double *partial = par ? fc.odd() : fc.even();
-                if (lat->dtype == GF_F64) {
+                if (variant == 7) {
+                    GaugeSweepArgs a{};
+                    a.U0 = (const float *)tiles.p; a.g = gb.p; a.links64 = 1; a.state = fc.st; a.partial = partial;
+                    a.geo = geo; a.parity = par; a.param = 1.7f; a.remap = 1;
+                    HIP_TRY(launch_gsweep(gauge, GF_OR, par + 1, true, a, s));
+                } else if (lat->dtype == GF_F64) {
                     SweepArgs<double> a{};
                     a.U = tiles.p; a.Udw = a.U; a.state = fc.st; a.g = geo; a.param = 1.7f; a.nsb = 32;
                     a.sweep = first + it; a.parity = par; a.partial = partial;
@@ -1314,8 +1326,10 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     if (!U || !out) return fail(GF_ERR_BAD_ARG, "null argument");
     if (lat->dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "the mixed driver takes an FP64 configuration");
     if (gauge < GF_LANDAU || gauge > GF_U1XU1) return fail(GF_ERR_BAD_ARG, "unknown gauge %d", gauge);
-    if (mode < GF_MIXED_LINKS || mode > GF_MIXED_GAUGE24) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
-    if (mode != GF_MIXED_LINKS) GF_TRY(check_gauge_sweep_volume(V));
+    if (mode < GF_MIXED_LINKS || mode > GF_MIXED_FP64) return fail(GF_ERR_BAD_ARG, "unknown mode %d", mode);
+    // GF_MIXED_FP64: no FP32 phase, the FP64 phase from the start
+    const bool fp32_phase = mode != GF_MIXED_FP64;
+    if (mode != GF_MIXED_LINKS && fp32_phase) GF_TRY(check_gauge_sweep_volume(V));
     if (gauge == GF_COULOMB && lat->dims[0] == 1)
         return fail(GF_ERR_BAD_ARG, "gf_run_mixed fixes every timeslice of a 4D lattice at once");
     if (gauge == GF_COULOMB && mode == GF_MIXED_LINKS)
@@ -1327,8 +1341,11 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     l32.layout = GF_LAYOUT_R12;
     const Geom geo = per_slice(make_geom(lat->dims, V), gauge);
     Scratch sU32, sg, sg64;
-    HIP_TRY(sU32.alloc(mode == GF_MIXED_GAUGE24 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
-    HIP_TRY(sg.alloc(sizeof(float) * gauge_floats(V), s));
+    if (fp32_phase) {
+        HIP_TRY(sU32.alloc(mode == GF_MIXED_GAUGE24 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32),
+                           s));
+        HIP_TRY(sg.alloc(sizeof(float) * gauge_floats(V), s));
+    }
     // the gauge modes' FP64 phase sweeps an FP64 g on the FP64 links (the
     // link mode, and lattices beyond the FP64 gauge sweep's offsets, sweep
     // the links themselves)
@@ -1352,11 +1369,13 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     } mg{mid};
     float *g = (float *)b.g;
     HIP_TRY(hipEventRecord(ev.a, s));
-    HIP_TRY(launch_gauge_identity(g, V, s));
-    if (mode == GF_MIXED_GAUGE24) HIP_TRY(launch_pack24(GF_F64, lat->layout, U, (uint32_t *)b.U32, V, s));
-    else HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
+    if (fp32_phase) {
+        HIP_TRY(launch_gauge_identity(g, V, s));
+        if (mode == GF_MIXED_GAUGE24) HIP_TRY(launch_pack24(GF_F64, lat->layout, U, (uint32_t *)b.U32, V, s));
+        else HIP_TRY(launch_convert(lat->dtype, lat->layout, U, GF_F32, 12, b.U32, V, 0, s));
+    }
     int done32 = 0;
-    if (mode == GF_MIXED_LINKS) {
+    if (fp32_phase && mode == GF_MIXED_LINKS) {
         // FP32 links and gauge transformation; every `refresh` sweeps the
         // working links are rebuilt from the FP64 configuration and the
         // (reprojected) transformation, which removes the FP32 drift of the
@@ -1373,7 +1392,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         };
         GF_TRY(fused_phase<float>(&l32, (float *)b.U32, g, gauge, omega, switch_precision, maxiter, check32,
                                   kCoarse * check32, kStall32, false, false, fc, s, after32, &done32, out));
-    } else {
+    } else if (fp32_phase) {
         // the links stay U0 (FP32 copy, read only); only g is swept, and the
         // kernel reprojects it in every update
         GaugeSweepArgs ga{};
@@ -1421,10 +1440,14 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         }
     }
     out->sweeps_fp32 = done32;
-    // switch: apply the accumulated transformation to the FP64 configuration,
-    // into the FP64 phase's tiles
-    HIP_TRY(launch_project_gauge(g, V, s));
-    HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, tiles.p, GF_F64, kTiled12, geo, s));
+    if (fp32_phase) {
+        // switch: apply the accumulated transformation to the FP64
+        // configuration, into the FP64 phase's tiles
+        HIP_TRY(launch_project_gauge(g, V, s));
+        HIP_TRY(launch_gauge_apply((const double *)U, lat->layout, g, tiles.p, GF_F64, kTiled12, geo, s));
+    } else {
+        HIP_TRY(launch_convert(GF_F64, lat->layout, U, GF_F64, kTiled12, tiles.p, V, 0, s));
+    }
     HIP_TRY(hipEventRecord(mid, s));
     auto after64 = [&](int it) -> gf_status {
         if (reproject > 0 && it % reproject == 0) return project_unless_done(&tiles.lat, tiles.p, fc.st, s);
@@ -1436,7 +1459,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     // an FP32 phase that went down to (near) the precision hands over a state
     // that may meet it after a sweep or two: check it before the even update
     // from the start
-    const bool near = switch_precision < kNearFactor * precision;
+    const bool near = fp32_phase && switch_precision < kNearFactor * precision;
     if (gauge64) {
         // FP64 sweeps of a second gauge transformation g64 (from 1) on the
         // links G U0 G^dagger, read only: the same OR updates as sweeping the

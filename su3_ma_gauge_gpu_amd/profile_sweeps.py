@@ -12,7 +12,9 @@ for kernel traces and hardware-counter runs:
 * `--sweeps` FP32 link sweeps with gauge accumulation on the compact FP32
   layout (the mixed driver's link mode);
 * `--sweeps` fused-check FP64 sweeps on the drivers' tiled copy of the links
-  (the FP64 kernel of gf_run_fast and of the mixed run's FP64 phase);
+  (the FP64 kernel of gf_run_fast and of the link mode's FP64 phase);
+* `--sweeps` fused-check FP64 gauge-transformation sweeps on the same tiled
+  links held read only (the FP64 phase of the mixed run's gauge modes);
 * `--sweeps` drop-in (reference-order) FP64 sweeps through gf_sweep on the
   18-real GpuPattern buffer;
 * `--sweeps` launches of the drop-in quality kernel (GaugeFixingStats'
@@ -62,6 +64,9 @@ def main() -> None:
     _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 0, C.byref(ms),
               None)
     print(f"fp64 fused sweep (tiled copy): {ms.value:.4f} ms per kernel", flush=True)
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 7, C.byref(ms),
+              None)
+    print(f"fp64 gauge-transformation sweep (tiled copy): {ms.value:.4f} ms per kernel", flush=True)
     K = gf.LandauKernelsSU3(lat64)
     for _ in range(args.sweeps):
         for parity in (0, 1):

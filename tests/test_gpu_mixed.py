@@ -243,7 +243,7 @@ def _returned_theta(oracle, gauge, dims, Ud):
 
 
 @pytest.mark.parametrize("gauge,prec", [(LANDAU, 1e-12), (COULOMB, 1e-12), (MAG, 1e-10)])
-@pytest.mark.parametrize("driver", ["fast", "mixed24", "mixed32", "links"])
+@pytest.mark.parametrize("driver", ["fast", "mixed24", "mixed32", "links", "fp64gauge"])
 def test_driver_returns_the_checked_state(gf, gpu, oracle, driver, gauge, prec):
     """The fused drivers hand back the configuration whose check met the
     precision, as the reference loop does (it breaks right after that check and
@@ -264,7 +264,8 @@ def test_driver_returns_the_checked_state(gf, gpu, oracle, driver, gauge, prec):
     if driver == "fast":
         res = gf.run_fast(lat, d, gauge, 1.7, prec, maxiter, 7)
     else:
-        mode = {"mixed24": gf.MIXED_GAUGE24, "mixed32": gf.MIXED_GAUGE32, "links": gf.MIXED_LINKS}[driver]
+        mode = {"mixed24": gf.MIXED_GAUGE24, "mixed32": gf.MIXED_GAUGE32, "links": gf.MIXED_LINKS,
+                "fp64gauge": gf.MIXED_FP64}[driver]
         res = gf.run_mixed(lat, d, gauge, 1.7, switch_precision=1e-6, precision=prec, maxiter=maxiter, mode=mode,
                            check32=5, reproject=7)
     assert res.converged == 1, res
@@ -390,6 +391,48 @@ def test_run_mixed_mag(gf, gpu, oracle, mode):
     assert res.converged == 1 and res.theta < 1e-10
     assert 0 < res.sweeps_fp32 < res.sweeps
     assert res.gff == pytest.approx(g, rel=1e-10)
+    assert _plaquette(to_matrices(d.download(), V), dims) == pytest.approx(_plaquette(to_matrices(U, V), dims),
+                                                                           rel=1e-13)
+
+
+@pytest.mark.parametrize("gauge,prec", [(LANDAU, 1e-12), (COULOMB, 1e-12), (MAG, 1e-10), (U1XU1, 1e-30)])
+def test_fp64_gauge_sweep_matches_fp64_link_sweeps(gf, gpu, oracle, gauge, prec):
+    """The all-FP64 run on the FP64 gauge-transformation sweep (run_mixed
+    without its FP32 phase: a gauge transformation g64 swept on the links held
+    read only, g64 U g64^dagger written back) makes the OR updates of the FP64
+    link sweep, rounded differently: the same sweep count to +-1 as run_fast
+    and the oracle's run_or, the oracle's functional to 1e-12, no FP32 sweep,
+    and the result a gauge transformation of the input (plaquette to 1e-13).
+    U1xU1 runs to a fixed sweep count (its theta need not converge) and is
+    compared with run_fast's links after the same sweeps."""
+    from oracle_lib import to_matrices
+    from test_gpu_parity import _plaquette
+
+    dims = (4, 6, 4, 8)
+    V = int(np.prod(dims))
+    U = _cfg(gf, dims)
+    lat = gf.Lattice(dims)
+    d = gf.DeviceBuffer.from_numpy(U)
+    maxiter = 400 if gauge == U1XU1 else 20000
+    res = gf.run_mixed(lat, d, gauge, 1.7, switch_precision=1e-6, precision=prec, maxiter=maxiter,
+                       mode=gf.MIXED_FP64, reproject=100)
+    d2 = gf.DeviceBuffer.from_numpy(U)
+    f = gf.run_fast(lat, d2, gauge, 1.7, prec, maxiter, 100)
+    print(gauge, res, f, flush=True)
+    assert res.sweeps_fp32 == 0
+    assert res.converged == f.converged
+    if gauge == U1XU1:
+        assert res.sweeps == f.sweeps == maxiter
+        assert np.abs(rows01(d.download(), V) - rows01(d2.download(), V)).max() < 1e-10
+    else:
+        assert res.converged == 1 and res.theta < prec
+        assert abs(res.sweeps - f.sweeps) <= 1
+        if gauge != COULOMB:
+            Uo = U.copy()
+            n, g, t = oracle.run_or(gauge, dims, Uo, 1.7, prec, maxiter, 1, 100)
+            assert abs(res.sweeps - n) <= 1
+            assert res.gff == pytest.approx(g, rel=1e-12)
+    assert res.gff == pytest.approx(f.gff, rel=1e-12)
     assert _plaquette(to_matrices(d.download(), V), dims) == pytest.approx(_plaquette(to_matrices(U, V), dims),
                                                                            rel=1e-13)
 

@@ -320,9 +320,12 @@ gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *s
 /* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
  * FP32 phase accumulates each site's gauge transformation g(x) until
  * theta < switch_precision (checked every check32 sweeps; the phase also ends
- * at the rounding floor of single precision: 8 checks in a row -- 25 for
- * MAG -- with theta below 1e-9 -- 1e-5 for MAG -- that do not lower its best
- * theta by 10%); then
+ * at the rounding floor of single precision: with theta below 1e-9, 8
+ * checks in a row that do not lower its best theta by 10%; for MAG, with
+ * theta below 1e-5, a window of 50 checks whose mean log(theta) drops by less
+ * than log(1.1) from the window before, and by less than log(1.001) or than
+ * a quarter of that window's own drop, so that slow but steady convergence
+ * goes on); then
  * U1 = G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
  * continue to theta < precision, checked every sweep; the array then holds

@@ -921,6 +921,12 @@ double gff_prefactor(int gauge) { return gauge == GF_COULOMB ? 1. / 9. : 1. / 12
 // rounding floor of FP32 sweeps lies near 1e-12 for the squared theta of
 // Landau, Coulomb and U1xU1, and higher for MAG's theta, a sum of |X|
 double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
+// how stagnation is judged (k_fused_check): Landau, Coulomb and U1xU1 switch
+// near the floor itself, where a phase that gains less than 10% in a few
+// checks has arrived; MAG switches far above its floor while converging by a
+// few percent per hundred sweeps at large volume, so its phase ends only when
+// the rate of descent itself collapses
+int stall_rule(int gauge) { return gauge == GF_MAG ? 1 : 0; }
 
 // One OR phase with the check fused into the sweeps.  `launch(parity,
 // theta, partial, it)` enqueues one parity kernel of sweep `it`;
@@ -931,8 +937,11 @@ double stall_floor(int gauge) { return gauge == GF_MAG ? 1e-5 : 1e-9; }
 // The end state of sweep `it` is checked when (it + 1) % check_every == 0:
 // its odd kernel forms the odd-site theta, the next even kernel the even-site
 // theta before updating, and a check kernel completes the sum.  The phase ends
-// once theta < tol or, with stall_limit > 0, after stall_limit checks in a row
-// that do not lower the smallest theta seen by 10% (r->converged = 2).
+// once theta < tol or, with stall_limit > 0, on stagnation (r->converged = 2):
+// below the gauge's stall floor, stall_limit checks in a row that do not lower
+// the best theta by 10%, or for MAG a window of stall_limit checks whose mean
+// log(theta) drops by less than log(1.1) and by far less than the window
+// before it (k_fused_check).
 // exact_end: the buffer must hold the state that met `tol` (the reference
 // breaks right after that check, LandauGaugeFixingSU3_4D.cu:254-271), not that
 // state plus the even half-sweep whose kernel formed the even sites' theta.
@@ -969,7 +978,8 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
     if (check_every < 1) check_every = 1;
     if (nblocks > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
     hipError_t e = launch_state_init(fc.st, tol, 1. / (3. * V), gff_prefactor(gauge) / V, stall_limit,
-                                     stall_floor(gauge), exact_end ? 1 : 0, kNearFactor * tol, near ? 1 : 0, s);
+                                     stall_floor(gauge), stall_rule(gauge), exact_end ? 1 : 0, kNearFactor * tol,
+                                     near ? 1 : 0, s);
     if (e != hipSuccess) return hip_fail(e, "state init");
     int it = 0;
     bool done = false;
@@ -1107,7 +1117,7 @@ gf_status final_quality_t(const gf_lattice *lat, R *U, int gauge, FusedCtx &fc, 
     const int64_t nb = sweep_grid(a.g);
     if (nb > fc.nblocks) return fail(GF_ERR_BAD_ARG, "internal: partial buffer too small");
     // tol 0 (never met), exact_end and near set: the passes run, the check reads them
-    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 1, 0.0, 1, s));
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 0, 1, 0.0, 1, s));
     for (int par = 0; par < 2; par++) {
         a.parity = par;
         a.partial = par ? fc.odd() : fc.even();
@@ -1242,7 +1252,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         HIP_TRY(launch_pack24(GF_F32, 12, U, pbuf.as<uint32_t>(), V, s));
     }
     // tol = 0: no state ever converges, every kernel does its full work
-    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 0, 0.0, 0, s));
+    HIP_TRY(launch_state_init(fc.st, 0.0, 1. / (3. * V), gff_prefactor(gauge) / V, 0, 0.0, 0, 0, 0.0, 0, s));
     EventPair ev;
     HIP_TRY(hipEventCreate(&ev.a));
     HIP_TRY(hipEventCreate(&ev.b));
@@ -1316,9 +1326,9 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
                        int maxiter, int mode, int refresh, int check32, int reproject, gf_run_result *out,
                        void *stream)
 {
-    // FP32 checks without a 10% gain in theta (below stall_floor) before
-    // switching anyway; MAG converges more slowly and gets a longer window
-    const int kStall32 = gauge == GF_MAG ? 25 : 8;
+    // FP32 checks per stagnation window (below stall_floor); MAG converges
+    // more slowly and gets a longer window
+    const int kStall32 = gauge == GF_MAG ? 50 : 8;
     // far above the switch threshold the FP32 phase checks 5x less often
     const int kCoarse = 5;
     int64_t V;
@@ -1440,6 +1450,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         }
     }
     out->sweeps_fp32 = done32;
+    const bool reached_switch = fp32_phase && out->converged == 1;
     if (fp32_phase) {
         // switch: apply the accumulated transformation to the FP64
         // configuration, into the FP64 phase's tiles
@@ -1458,8 +1469,9 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     const int left = std::max(0, maxiter - done32);
     // an FP32 phase that went down to (near) the precision hands over a state
     // that may meet it after a sweep or two: check it before the even update
-    // from the start
-    const bool near = fp32_phase && switch_precision < kNearFactor * precision;
+    // from the start (not after a stagnated phase, whose theta is further off:
+    // the device sets `near` once a check comes close)
+    const bool near = reached_switch && switch_precision < kNearFactor * precision;
     if (gauge64) {
         // FP64 sweeps of a second gauge transformation g64 (from 1) on the
         // links G U0 G^dagger, read only: the same OR updates as sweeping the

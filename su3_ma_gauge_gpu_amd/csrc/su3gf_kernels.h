@@ -15,10 +15,15 @@ struct DevState {
     int converged_sweep;  // index of the sweep whose end state met tol (or stagnated)
     int checked_sweep;    // index of the sweep whose end state `theta` describes
     int checks;           // number of completed checks
-    int stall;            // consecutive checks without halving the best theta
-    int stall_limit;      // 0: no stagnation exit
+    int stall;            // rule 0: checks in a row without a 10% gain; 1: checks in the current window
+    int stall_limit;      // that many checks stagnate (rule 0) or make a window (rule 1); 0: no stagnation exit
     double stall_below;   // stagnation is only tested for theta below this
-    double best;          // smallest theta seen
+    int stall_rule;       // 0: the best theta's gain; 1: windows' mean log(theta), for slow convergence
+    double best;          // rule 0: the best theta, lowered in steps of 10%
+    int windows;          // rule 1: windows completed below stall_below
+    double win_logsum;    // sum of log(theta) over the current window
+    double win_mean;      // mean of log(theta) over the previous window
+    double win_drop;      // the previous window's decrease of that mean
     double tol;           // theta threshold
     double theta_scale;   // 1 / (Nc * V)
     double gff_scale;     // 1 / (Nc * Ndim * V)
@@ -171,7 +176,8 @@ hipError_t launch_gauge_identity64(double *g, int64_t V, hipStream_t s);
 hipError_t launch_gauge_apply64(const double *U1, const double *g, double *out, int so, const Geom &geo, int reun,
                                 hipStream_t s);
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below, int exact_end, double near_below, int near, hipStream_t s);
+                             double stall_below, int stall_rule, int exact_end, double near_below, int near,
+                             hipStream_t s);
 // theta of the state after sweep `sweep - 1` from the even partials of sweep
 // `sweep` (pre-update) and the odd partials of sweep `sweep - 1` (post-update);
 // nseg > 1: the blocks form nseg independent problems (timeslices), theta is

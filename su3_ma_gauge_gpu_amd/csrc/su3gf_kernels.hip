@@ -310,7 +310,7 @@ hipError_t launch_gauge_apply64(const double *U1, const double *g, double *out, 
 
 // ------------------------------------------------------------ driver state
 __global__ void k_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below, int exact_end, double near_below, int near)
+                             double stall_below, int stall_rule, int exact_end, double near_below, int near)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     st->done = 0;
@@ -320,7 +320,12 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
     st->stall = 0;
     st->stall_limit = stall_limit;
     st->stall_below = stall_below;
+    st->stall_rule = stall_rule;
     st->best = 1e300;
+    st->windows = 0;
+    st->win_logsum = 0;
+    st->win_mean = 0;
+    st->win_drop = 0;
     st->tol = tol;
     st->theta_scale = theta_scale;
     st->gff_scale = gff_scale;
@@ -332,10 +337,11 @@ __global__ void k_state_init(DevState *st, double tol, double theta_scale, doubl
 }
 
 hipError_t launch_state_init(DevState *st, double tol, double theta_scale, double gff_scale, int stall_limit,
-                             double stall_below, int exact_end, double near_below, int near, hipStream_t s)
+                             double stall_below, int stall_rule, int exact_end, double near_below, int near,
+                             hipStream_t s)
 {
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, theta_scale, gff_scale, stall_limit,
-                       stall_below, exact_end, near_below, near);
+                       stall_below, stall_rule, exact_end, near_below, near);
     return hipGetLastError();
 }
 
@@ -360,23 +366,53 @@ __device__ __forceinline__ void check_update(DevState *st, double theta, double 
     st->gff = gff;
     st->checked_sweep = sweep - 1;
     st->checks++;
-    // stagnation: stall_limit checks in a row, all with theta below
-    // stall_below, without lowering the best theta by 10%.  Single-precision
-    // sweeps that have reached their rounding floor gain nothing; above
-    // stall_below theta may still rise for a while (overrelaxation is not
-    // monotone, and a Coulomb lattice tests its worst slice) and is not tested.
-    if (theta < 0.9 * st->best) {
-        st->best = theta;
-        st->stall = 0;
-    } else if (theta < st->stall_below) {
-        st->stall++;
+    // stagnation, tested only for theta below stall_below (above it theta may
+    // still rise for a while: overrelaxation is not monotone, and a Coulomb
+    // lattice tests its worst slice)
+    bool stalled = false;
+    if (st->stall_rule == 0) {
+        // stall_limit checks in a row without lowering the best theta by 10%:
+        // single-precision sweeps at their rounding floor gain nothing
+        if (theta < 0.9 * st->best) {
+            st->best = theta;
+            st->stall = 0;
+        } else if (theta < st->stall_below) {
+            st->stall++;
+        } else {
+            st->stall = 0;
+        }
+        stalled = st->stall_limit > 0 && st->stall >= st->stall_limit;
+    } else if (theta < st->stall_below && theta > 0) {
+        // windows of stall_limit checks, for runs that converge slowly (MAG at
+        // large volume: a few percent per window) far above their rounding
+        // floor.  A window's drop is the decrease of the mean of log(theta)
+        // from the window before; the run has stagnated when a drop is below
+        // log(1.1) and either below log(1.001) or below a quarter of the
+        // previous drop, so a steady slow rate goes on.  Means rather than
+        // minima keep the check-to-check scatter of theta from passing for a
+        // change of rate.
+        st->win_logsum += log(theta);
+        if (++st->stall >= st->stall_limit && st->stall_limit > 0) {
+            const double mean = st->win_logsum / st->stall;
+            if (st->windows > 0) {
+                const double drop = st->win_mean - mean;
+                stalled = drop < log(1.1) && (drop < log(1.001) || (st->windows > 1 && drop < 0.25 * st->win_drop));
+                st->win_drop = drop;
+            }
+            st->win_mean = mean;
+            st->windows++;
+            st->stall = 0;
+            st->win_logsum = 0;
+        }
     } else {
         st->stall = 0;
+        st->win_logsum = 0;
+        st->windows = 0;
     }
     if (theta < st->tol && !(updated && st->exact_end)) {
         st->done = 1;
         st->converged_sweep = sweep - 1;
-    } else if (st->stall_limit > 0 && st->stall >= st->stall_limit) {
+    } else if (stalled) {
         st->done = 2;
         st->converged_sweep = sweep - 1;
     }

@@ -86,6 +86,17 @@ def test_gauge_sweep_volume_limit(gf):
     ms = C.c_float()
     for variant in (2, 3, 4):
         assert lib.gf_time_sweeps(C.byref(big32), good, 0, 10, variant, C.byref(ms), None) == 1, variant
+    # the FP64 gauge-transformation sweep addresses 12 doubles per site with
+    # 32-bit offsets: at most 22,369,600 sites, an FP64 lattice only
+    big64 = _lat(gf, (70, 70, 70, 70))  # 24,010,000 sites
+    assert lib.gf_time_sweeps(C.byref(big64), good, 0, 10, 7, C.byref(ms), None) == 1
+    assert "22,369,600" in gf._lib.last_error()
+    assert lib.gf_time_sweeps(C.byref(_lat(gf, (8, 8, 8, 8), 0, 12)), good, 0, 10, 7, C.byref(ms), None) == 2
+    assert lib.gf_time_sweeps(C.byref(_lat(gf, (8, 8, 8, 8))), good, 0, 10, 8, C.byref(ms), None) == 2
+    # modes past GF_MIXED_FP64 are unknown
+    small = _lat(gf, (8, 8, 8, 8))
+    assert lib.gf_run_mixed(C.byref(small), good, 0, 1.7, 1e-6, 1e-12, 10, 5, 100, 10, 100, C.byref(out), None) == 2
+    assert "unknown mode" in gf._lib.last_error()
 
 
 def test_batch_calls_check_every_problem_on_the_host(gf):

@@ -221,6 +221,12 @@ def coulomb_line(args) -> dict:
     rs = [mixed() for _ in range(args.steps)]
     r = rs[-1]
     th_m, g_m = slice_quality(lat, work, stream)
+    # the workload's stated schedule: FP32 12-real links until theta < 1e-6,
+    # then FP64 sweeps (on the gauge transformation)
+    reset()
+    st = gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.early_switch, args.precision, args.maxiter,
+                      mode=gf.MIXED_GAUGE32, check32=args.check32, stream=stream)
+    th_s, g_s = slice_quality(lat, work, stream)
     reset()
     f = gf.run_fast(lat, work, gf.COULOMB, args.omega, args.precision, args.maxiter, args.reproject, stream=stream)
     th_f, g_f = slice_quality(lat, work, stream)
@@ -237,6 +243,10 @@ def coulomb_line(args) -> dict:
         "max_slice_theta_mixed": float(th_m.max()), "max_slice_theta_fp64": float(th_f.max()),
         "max_slice_gff_rel_diff": float(np.max(np.abs(g_m - g_f) / np.abs(g_f))),
         "gff_mixed": r.gff, "gff_fp64": f.gff,
+        "mixed_fp32_links_switch_1e-6": {
+            "time_to_theta_s": st.ms_total / 1e3, "sweeps": st.sweeps, "sweeps_fp32": st.sweeps_fp32,
+            "speedup_vs_fp64": f.ms_total / st.ms_total, "max_slice_theta": float(th_s.max()),
+            "max_slice_gff_rel_diff": float(np.max(np.abs(g_s - g_f) / np.abs(g_f)))},
     })
     return line
 

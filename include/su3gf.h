@@ -317,6 +317,23 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
 gf_status gf_set_wavefront(int slab, int *previous);
 gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab);
 
+/* The gauge-transformation sweep's block order across the eight XCDs (each
+ * with its own L2): each XCD takes a contiguous range of timeslices and walks
+ * it with t innermost, so that a block's t neighbours were read by the block
+ * before it.  Walking the range in passes of n timeslices brings a block's x
+ * neighbours closer in that sequence, at the cost of the t neighbours across
+ * a pass boundary.  The results do not depend on the order (the sites of a
+ * parity are independent); the time does.  Chosen per lattice shape, gauge
+ * and mode by timing, together with the wavefront slab.
+ * gf_set_block_order: -1 choose by timing (default), 0 one pass, n > 0 passes
+ * of n timeslices (where n divides the XCD's range; otherwise one pass);
+ * *previous (optional) receives the old setting.
+ * gf_block_order_choice: the order in use for gf_run_mixed of lat with
+ * `gauge` and gauge-sweep `mode` (-1 not chosen yet).
+ * No reference counterpart. */
+gf_status gf_set_block_order(int tpass, int *previous);
+gf_status gf_block_order_choice(const gf_lattice *lat, int gauge, int mode, int *tpass);
+
 /* Mixed precision.  U (FP64, layout of lat) holds the configuration U0; an
  * FP32 phase accumulates each site's gauge transformation g(x) until
  * theta < switch_precision (checked every check32 sweeps; the phase also ends

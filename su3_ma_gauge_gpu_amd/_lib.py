@@ -104,6 +104,8 @@ SIGNATURES = {
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
     "gf_set_wavefront": (_i, [_i, C.POINTER(_i)]),
     "gf_wavefront_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
+    "gf_set_block_order": (_i, [_i, C.POINTER(_i)]),
+    "gf_block_order_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),
     "gf_batch_run_mixed": (_i, [_i, _pl, C.POINTER(_vp), _i, _f, _f, _f, _i, _i, _i, _i, _i, _i, C.POINTER(_vp),

@@ -608,13 +608,29 @@ gf_status gf_run_or(const gf_lattice *lat, void *U, int gauge, float omega, floa
 // bit-identical results, so only the time depends on it.
 namespace {
 std::mutex g_slab_mu;
-std::map<std::array<int, 8>, int> g_slab_cache;
-int g_slab_setting = -1;  // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
+// per (shape, gauge, link form, arithmetic): the wavefront slab of sweeps
+// without a check (0: two kernels) and the XCD block order (tpass: passes of
+// that many timeslices, 0: one pass)
+struct GsweepTuning {
+    int slab = 0;
+    int tpass = 0;
+};
+std::map<std::array<int, 8>, GsweepTuning> g_slab_cache;
+int g_slab_setting = -1;   // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
+int g_tpass_setting = -1;  // -1 tuned, 0 one pass, n > 0 passes of n timeslices (gf_set_block_order)
 
 // links: 0 FP32, 1 24-bit, 2 FP64 (with an FP64 g)
 std::array<int, 8> slab_key(const Geom &geo, int gauge, int links, bool fp64)
 {
     return {geo.N[0], geo.N[1], geo.N[2], geo.N[3], gauge, links, fp64 ? 1 : 0, geo.slice_bps ? 1 : 0};
+}
+
+// timeslices each XCD walks in the launcher's timeslice order (launch_gsweep's
+// remap = 2), or 0 where that order does not apply
+int xcd_slices(const Geom &geo)
+{
+    const int64_t half_slice = geo.Vh / geo.N[0];
+    return geo.N[0] % 8 == 0 && half_slice % 64 == 0 ? geo.N[0] / 8 : 0;
 }
 
 // average ms of one sweep over `n` sweeps (after one unmeasured), slab 0 = two kernels
@@ -642,46 +658,75 @@ float time_gsweep(const GaugeSweepArgs &t, int gauge, bool fp64, int slab, int n
     return ms / n;
 }
 
+// times a few OR sweeps of each candidate on a scratch copy of g: first the
+// block order of the two-kernel sweep (passes of each divisor of the XCD's
+// timeslices), then, with that order, the wavefront slabs; a candidate must
+// beat the incumbent by 3% (launch count, noise).  Never fails (the defaults
+// on any error).
+GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
+{
+    const Geom &geo = ga.geo;
+    GsweepTuning tu;
+    Scratch g;
+    EventPair ev;
+    const size_t gbytes = (ga.links64 ? sizeof(double) : sizeof(float)) * gauge_floats(geo.V);
+    if (g.alloc(gbytes, s) == hipSuccess && hipMemcpyAsync(g.p, ga.g, gbytes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+        hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
+        GaugeSweepArgs t = ga;
+        t.g = g.as<float>();
+        t.state = nullptr;
+        t.partial = nullptr;
+        t.param = 1.7f;
+        t.tpass = g_tpass_setting >= 0 ? g_tpass_setting : 0;
+        float two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+        const int spx = xcd_slices(geo);
+        if (g_tpass_setting < 0 && spx > 1 && two > 0) {
+            const float one_pass = two;
+            for (int P = 1; P < spx; P++) {
+                if (spx % P != 0) continue;
+                t.tpass = P;
+                const float ms = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+                if (ms > 0 && ms < two && ms < 0.97f * one_pass) {
+                    two = ms;
+                    tu.tpass = P;
+                }
+            }
+            t.tpass = tu.tpass;
+        }
+        float best_ms = two;
+        for (int c : {2, 3, 4, 6, 8, 12}) {
+            if (c >= geo.N[0] || two <= 0) break;
+            const float ms = time_gsweep(t, gauge, fp64, c, 3, s, ev.a, ev.b);
+            if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
+                best_ms = ms;
+                tu.slab = c;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    return tu;
+}
+
 // the slab for sweeps without a check of the gauge-transformation sweep
-// described by ga (0: the two-kernel sweep); never fails (0 on any error)
-int wavefront_slab(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
+// described by ga (0: the two-kernel sweep), and its block order, set in
+// ga.tpass; tuned on the first call for a kind of lattice unless both are set
+// (gf_set_wavefront, gf_set_block_order).  Never fails.
+int wavefront_slab(GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
 {
     const Geom &geo = ga.geo;
     std::lock_guard<std::mutex> lock(g_slab_mu);
-    if (g_slab_setting >= 0) return g_slab_setting;
-    if (geo.N[0] < 4) return 0;
+    GsweepTuning tu;
     const auto key = slab_key(geo, gauge, ga.links64 ? 2 : ga.links24, fp64);
     const auto hit = g_slab_cache.find(key);
-    if (hit != g_slab_cache.end()) return hit->second;
-    int best = 0;
-    {
-        Scratch g;
-        EventPair ev;
-        const size_t gbytes = (ga.links64 ? sizeof(double) : sizeof(float)) * gauge_floats(geo.V);
-        if (g.alloc(gbytes, s) == hipSuccess && hipMemcpyAsync(g.p, ga.g, gbytes, hipMemcpyDeviceToDevice, s) == hipSuccess &&
-            hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
-            GaugeSweepArgs t = ga;
-            t.g = g.as<float>();
-            t.state = nullptr;
-            t.partial = nullptr;
-            t.param = 1.7f;
-            const float two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
-            float best_ms = two;
-            for (int c : {2, 3, 4, 6, 8, 12}) {
-                if (c >= geo.N[0] || two <= 0) break;
-                const float ms = time_gsweep(t, gauge, fp64, c, 3, s, ev.a, ev.b);
-                // a wavefront must beat the two kernels by 3% (launch count, noise)
-                if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
-                    best_ms = ms;
-                    best = c;
-                }
-            }
-        }
-        (void)hipStreamSynchronize(s);
-        (void)hipGetLastError();
+    if (hit != g_slab_cache.end()) {
+        tu = hit->second;
+    } else if (geo.N[0] >= 4 && (g_slab_setting < 0 || g_tpass_setting < 0)) {
+        tu = tune_gsweep(ga, gauge, fp64, s);
+        g_slab_cache[key] = tu;
     }
-    g_slab_cache[key] = best;
-    return best;
+    ga.tpass = g_tpass_setting >= 0 ? g_tpass_setting : tu.tpass;
+    return g_slab_setting >= 0 ? g_slab_setting : tu.slab;
 }
 }  // namespace
 
@@ -694,20 +739,48 @@ extern "C" gf_status gf_set_wavefront(int slab, int *previous)
     return GF_OK;
 }
 
-extern "C" gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab)
+extern "C" gf_status gf_set_block_order(int tpass, int *previous)
+{
+    if (tpass < -1) return fail(GF_ERR_BAD_ARG, "block order setting %d (-1 tuned, 0 one pass, n > 0 fixed)", tpass);
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (previous) *previous = g_tpass_setting;
+    g_tpass_setting = tpass;
+    return GF_OK;
+}
+
+namespace {
+gf_status tuning_of(const gf_lattice *lat, int gauge, int mode, GsweepTuning *tu, bool *known)
 {
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
-    if (!slab) return fail(GF_ERR_BAD_ARG, "slab is null");
     if (mode < GF_MIXED_GAUGE32 || mode > GF_MIXED_GAUGE24) return fail(GF_ERR_BAD_ARG, "not a gauge-sweep mode: %d", mode);
     const Geom geo = per_slice(make_geom(lat->dims, V), gauge);
-    std::lock_guard<std::mutex> lock(g_slab_mu);
-    if (g_slab_setting >= 0) {
-        *slab = g_slab_setting;
-        return GF_OK;
-    }
     const auto hit = g_slab_cache.find(slab_key(geo, gauge, mode == GF_MIXED_GAUGE24, mode == GF_MIXED_GAUGE64));
-    *slab = hit == g_slab_cache.end() ? -1 : hit->second;
+    *known = hit != g_slab_cache.end();
+    if (*known) *tu = hit->second;
+    return GF_OK;
+}
+}  // namespace
+
+extern "C" gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab)
+{
+    if (!slab) return fail(GF_ERR_BAD_ARG, "slab is null");
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    GsweepTuning tu;
+    bool known = false;
+    GF_TRY(tuning_of(lat, gauge, mode, &tu, &known));
+    *slab = g_slab_setting >= 0 ? g_slab_setting : known ? tu.slab : -1;
+    return GF_OK;
+}
+
+extern "C" gf_status gf_block_order_choice(const gf_lattice *lat, int gauge, int mode, int *tpass)
+{
+    if (!tpass) return fail(GF_ERR_BAD_ARG, "tpass is null");
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    GsweepTuning tu;
+    bool known = false;
+    GF_TRY(tuning_of(lat, gauge, mode, &tu, &known));
+    *tpass = g_tpass_setting >= 0 ? g_tpass_setting : known ? tu.tpass : -1;
     return GF_OK;
 }
 
@@ -958,10 +1031,13 @@ int stall_rule(int gauge) { return gauge == GF_MAG ? 1 : 0; }
 int check_segments(const int dims[4], int gauge) { return gauge == GF_COULOMB && dims[0] > 1 ? dims[0] : 1; }
 
 // coarse_every > check_every: while the last theta the host has read is above
-// kCoarseFactor * tol (or before the first check), check only every
-// coarse_every sweeps.  Such a theta cannot reach tol within the next chunk,
-// and stagnation is tested far below it, so only the cost of the checks
-// changes.  0 checks every check_every sweeps throughout.
+// kCoarseFactor * tol, check only every coarse_every sweeps.  Such a theta
+// cannot reach tol within the next chunk, and stagnation is tested far below
+// it, so only the cost of the checks changes.  0 checks every check_every
+// sweeps throughout.  coarse_start: the phase also checks coarsely before its
+// first read (a phase that starts far from tol) and turns fine for good at the
+// first theta near tol; otherwise it starts fine and follows the theta read
+// after each chunk.
 constexpr double kCoarseFactor = 1e3;
 constexpr double kNearFactor = 2.0;
 constexpr double kHostNearFactor = 1e3;
@@ -972,7 +1048,7 @@ template <typename Launch, typename After, typename Plain>
 gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int maxiter, int check_every,
                      int coarse_every, int stall_limit, bool exact_end, bool near, int64_t nblocks, FusedCtx &fc,
                      hipStream_t s, Launch launch, After after_sweep, Plain plain, int *sweeps_done,
-                     gf_run_result *r)
+                     gf_run_result *r, bool coarse_start = true)
 {
     const int nseg = check_segments(dims, gauge);
     if (check_every < 1) check_every = 1;
@@ -984,7 +1060,7 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
     int it = 0;
     bool done = false;
     bool pending = false;  // a check of the state at the end of sweep it-1 is due
-    bool coarse = coarse_every > check_every;
+    bool coarse = coarse_every > check_every && coarse_start;
     bool host_near = exact_end;  // until the first read of this phase's state
     auto check = [&](int mode, int sweep) -> gf_status {
         const hipError_t ec = launch_fused_check(fc.st, fc.even(), fc.odd(), nblocks, nseg, sweep, mode, s);
@@ -1024,7 +1100,11 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
         }
         GF_TRY(fc.read(s));
         done = fc.host->done != 0;
-        if (coarse && fc.host->checks > 0 && !(fc.host->theta > kCoarseFactor * tol)) coarse = false;
+        if (coarse_every > check_every && fc.host->checks > 0) {
+            const bool far = fc.host->theta > kCoarseFactor * tol;
+            if (!coarse_start) coarse = far;  // coarse only while a theta the host has read is far
+            else if (!far) coarse = false;    // coarse until the first theta near tol
+        }
         if (exact_end && fc.host->checks > 0) host_near = fc.host->near || !(fc.host->theta > kHostNearFactor * tol);
     }
     // the host copy of the state is this loop's only after a chunk has run
@@ -1046,7 +1126,7 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
 template <typename R, typename After>
 gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float omega, double tol, int maxiter,
                       int check_every, int coarse_every, int stall_limit, bool exact_end, bool near, FusedCtx &fc,
-                      hipStream_t s, After after_sweep, int *sweeps_done, gf_run_result *r)
+                      hipStream_t s, After after_sweep, int *sweeps_done, gf_run_result *r, bool coarse_start = true)
 {
     int64_t V = 1;
     for (int i = 0; i < 4; i++) V *= lat->dims[i];
@@ -1066,7 +1146,7 @@ gf_status fused_phase(const gf_lattice *lat, R *U, float *gacc, int gauge, float
     };
     auto two_kernels = [](int) { return hipErrorNotSupported; };
     return fused_loop(lat->dims, V, gauge, tol, maxiter, check_every, coarse_every, stall_limit, exact_end, near,
-                      sweep_grid(a.g), fc, s, launch, after_sweep, two_kernels, sweeps_done, r);
+                      sweep_grid(a.g), fc, s, launch, after_sweep, two_kernels, sweeps_done, r, coarse_start);
 }
 
 // projectSU3 enqueued by a driver after a sweep: skipped on the device once
@@ -1263,6 +1343,14 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
         wa.U0 = pbuf.as<const float>(); wa.g = gb.p; wa.geo = geo; wa.param = 1.7f; wa.remap = 1; wa.links24 = 1;
         slab = wavefront_slab(wa, gauge, false, s);
     }
+    int tpass = 0;  // the gauge-transformation sweeps' block order, as the drivers use it
+    if ((variant >= 2 && variant <= 4) || variant == 7) {
+        GaugeSweepArgs t{};
+        t.U0 = variant == 7 ? (const float *)tiles.p : variant == 4 ? pbuf.as<const float>() : (const float *)U;
+        t.g = gb.p; t.geo = geo; t.param = 1.7f; t.remap = 1; t.links24 = variant == 4; t.links64 = variant == 7;
+        (void)wavefront_slab(t, gauge, variant == 3 || variant == 7, s);
+        tpass = t.tpass;
+    }
     auto run = [&](int n, int first) -> gf_status {
         for (int it = 0; it < n; it++) {
             if (variant == 6) {
@@ -1283,7 +1371,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
                 if (variant == 7) {
                     GaugeSweepArgs a{};
                     a.U0 = (const float *)tiles.p; a.g = gb.p; a.links64 = 1; a.state = fc.st; a.partial = partial;
-                    a.geo = geo; a.parity = par; a.param = 1.7f; a.remap = 1;
+                    a.geo = geo; a.parity = par; a.param = 1.7f; a.remap = 1; a.tpass = tpass;
                     HIP_TRY(launch_gsweep(gauge, GF_OR, par + 1, true, a, s));
                 } else if (lat->dtype == GF_F64) {
                     SweepArgs<double> a{};
@@ -1301,7 +1389,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
                     const bool q24 = variant == 4;
                     a.U0 = q24 ? pbuf.as<const float>() : (const float *)U;
                     a.g = gb.p; a.state = fc.st; a.partial = partial; a.geo = geo;
-                    a.parity = par; a.param = 1.7f; a.remap = 1; a.links24 = q24;
+                    a.parity = par; a.param = 1.7f; a.remap = 1; a.links24 = q24; a.tpass = tpass;
                     HIP_TRY(launch_gsweep(gauge, GF_OR, 0, variant == 3, a, s));
                 }
             }
@@ -1472,6 +1560,11 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     // from the start (not after a stagnated phase, whose theta is further off:
     // the device sets `near` once a check comes close)
     const bool near = reached_switch && switch_precision < kNearFactor * precision;
+    // MAG's checks cost a third of a sweep in this phase (its theta is formed
+    // from the links in every wave); far from the target it converges by a few
+    // percent per hundred sweeps, so it checks every 5 sweeps while a theta
+    // the host has read is above kCoarseFactor * precision
+    const int coarse64 = gauge == GF_MAG ? 5 : 0;
     if (gauge64) {
         // FP64 sweeps of a second gauge transformation g64 (from 1) on the
         // links G U0 G^dagger, read only: the same OR updates as sweeping the
@@ -1498,15 +1591,15 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
             return launch_gsweep_wavefront(gauge, GF_OR, true, gd, slab64, s);
         };
         auto none = [](int) -> gf_status { return GF_OK; };
-        GF_TRY(fused_loop(lat->dims, V, gauge, precision, left, 1, 0, 0, true, near, sweep_grid(geo), fc, s, launch64,
-                          none, plain64, &done64, out));
+        GF_TRY(fused_loop(lat->dims, V, gauge, precision, left, 1, coarse64, 0, true, near, sweep_grid(geo), fc, s,
+                          launch64, none, plain64, &done64, out, false));
         out->sweeps = done32 + done64;
         // U = G64 (G U0 G^dagger) G64^dagger into the caller's links, projected
         // (third rows, LandauGaugeFixingSU3_4D.cu:271)
         HIP_TRY(launch_gauge_apply64(tiles.p, g64, (double *)U, lat->layout, geo, 1, s));
     } else {
-        GF_TRY(fused_phase<double>(&tiles.lat, tiles.p, nullptr, gauge, omega, precision, left, 1, 0, 0, true, near,
-                                   fc, s, after64, &done64, out));
+        GF_TRY(fused_phase<double>(&tiles.lat, tiles.p, nullptr, gauge, omega, precision, left, 1, coarse64, 0, true,
+                                   near, fc, s, after64, &done64, out, false));
         out->sweeps = done32 + done64;
         // back to the caller's links, projected (third rows, LandauGaugeFixingSU3_4D.cu:271)
         HIP_TRY(launch_convert(GF_F64, kTiled12, tiles.p, GF_F64, lat->layout, U, V, 1, s));

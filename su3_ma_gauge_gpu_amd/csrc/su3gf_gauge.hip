@@ -449,15 +449,22 @@ template <typename A> __device__ __forceinline__ void theta_from_m(const M3<A> &
 // timeslice of one parity a whole number of 64-site blocks) also walks each
 // XCD's eighth -- Nt/8 timeslices -- with t innermost, so that the t
 // neighbours' g is read from L2 too: in plain order they are a timeslice
-// (V3/2 sites, megabytes of traffic) apart.
+// (V3/2 sites, megabytes of traffic) apart.  With t innermost, a block's x
+// neighbours are (blocks per timeslice row of x) x (timeslices walked
+// together) blocks away in the XCD's sequence; walking the XCD's timeslices in
+// passes of tpass shortens that distance (more x neighbours in L2) at the
+// cost of the t neighbours across a pass boundary.
 __device__ __forceinline__ int64_t logical_block(const GaugeSweepArgs &a)
 {
     const uint32_t b = blockIdx.x, n = gridDim.x;  // below 2^31
     if (a.remap == 0) return b;
     const uint32_t xcd = b & 7, k = b >> 3;
     if (a.remap == 1) return (int64_t)xcd * (n >> 3) + k;
-    const uint32_t q = fdiv(k, a.spx_div);
-    const int64_t t = (int64_t)xcd * a.slices_per_xcd + (k - q * (uint32_t)a.slices_per_xcd);  // timeslice
+    const uint32_t P = (uint32_t)a.tpass;
+    const uint32_t pass = fdiv(k, a.pass_div);
+    const uint32_t rem = k - pass * P * (uint32_t)a.slice_blocks;
+    const uint32_t q = fdiv(rem, a.tpass_div);
+    const int64_t t = (int64_t)xcd * a.slices_per_xcd + pass * P + (rem - q * P);  // timeslice
     return t * a.slice_blocks + q;
 }
 
@@ -1020,7 +1027,9 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
         b.remap = 2;
         b.slice_blocks = half_slice / 64;
         b.slices_per_xcd = a.geo.N[0] / 8;
-        b.spx_div = make_fastdiv((uint32_t)b.slices_per_xcd);
+        if (b.tpass <= 0 || b.slices_per_xcd % b.tpass != 0) b.tpass = b.slices_per_xcd;
+        b.tpass_div = make_fastdiv((uint32_t)b.tpass);
+        b.pass_div = make_fastdiv((uint32_t)(b.tpass * b.slice_blocks));
     } else if (b.remap && sweep_grid(a.geo) % 8 == 0) {
         b.remap = 1;
     } else {

@@ -89,8 +89,10 @@ struct GaugeSweepArgs {
     float param;       // OR parameter or SA temperature
     int remap;         // block order: 0 plain, else chosen by the launcher (XCD-aware)
     int slices_per_xcd;  // set by the launcher for remap = 2
-    FastDiv spx_div;     // division by slices_per_xcd
     int64_t slice_blocks;
+    int tpass;           // remap = 2: each XCD walks its timeslices in passes of tpass (a divisor of
+                         // slices_per_xcd; 0 = all of them in one pass), t innermost within a pass
+    FastDiv tpass_div, pass_div;  // division by tpass and by tpass * slice_blocks
     uint32_t seed, counter;  // SA / random transformation (Philox key and counter)
     int64_t nsb;       // sites per block of the reference launch (RNG key)
     int links24;       // U0 in the 24-bit fixed-point form of launch_pack24 (OR, SA, microcanonical)

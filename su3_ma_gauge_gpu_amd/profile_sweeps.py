@@ -20,8 +20,9 @@ for kernel traces and hardware-counter runs:
 * `--sweeps` launches of the drop-in quality kernel (GaugeFixingStats'
   per-site pass) on the same buffer.
 
-Usage: python su3_ma_gauge_gpu_amd/profile_sweeps.py [--dims 32 32 32 32] [--sweeps 10]
-(--dims 48 48 48 48: a working set far beyond the 256 MB Infinity Cache)
+Usage: python su3_ma_gauge_gpu_amd/profile_sweeps.py [--dims 32 32 32 32] [--sweeps 10] [--gauge landau|mag]
+(--dims 48 48 48 48: a working set far beyond the 256 MB Infinity Cache;
+--gauge mag: the same kernels with MAG's update and check)
 """
 from __future__ import annotations
 
@@ -40,11 +41,14 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--dims", type=int, nargs=4, default=[32, 32, 32, 32])
     ap.add_argument("--sweeps", type=int, default=10)
+    ap.add_argument("--gauge", choices=["landau", "mag"], default="landau")
     args = ap.parse_args()
     dims = tuple(args.dims)
+    gauge = gf.MAG if args.gauge == "mag" else gf.LANDAU
+    kernels = gf.MAGKernelsSU3 if args.gauge == "mag" else gf.LandauKernelsSU3
     lat64 = gf.Lattice(dims, gf.F64, gf.PATTERN18)
     lat32 = gf.Lattice(dims, gf.F32, gf.R12)
-    U0 = gf.make_config(dims, 0.25, 1, 2)
+    U0 = gf.make_config(dims, 0.35 if args.gauge == "mag" else 0.25, 1, 2)
     src = gf.DeviceBuffer.from_numpy(U0)
     work = gf.DeviceBuffer(lat64.nbytes)
     for _ in range(3):
@@ -58,22 +62,22 @@ def main() -> None:
                           (4, "fp32 gauge-transformation sweep, 24-bit fixed-point links"),
                           (6, "the same, each sweep as the mixed run's FP32 phase runs it (per half sweep)"),
                           (1, "fp32 link sweep with gauge accumulation")):
-        _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gf.LANDAU, args.sweeps, variant,
+        _lib.call("gf_time_sweeps", C.byref(lat32.struct), C.c_void_p(u32.ptr), gauge, args.sweeps, variant,
                   C.byref(ms), None)
         print(f"{what}: {ms.value:.4f} ms per kernel", flush=True)
-    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 0, C.byref(ms),
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gauge, args.sweeps, 0, C.byref(ms),
               None)
     print(f"fp64 fused sweep (tiled copy): {ms.value:.4f} ms per kernel", flush=True)
-    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 7, C.byref(ms),
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gauge, args.sweeps, 7, C.byref(ms),
               None)
     print(f"fp64 gauge-transformation sweep (tiled copy): {ms.value:.4f} ms per kernel", flush=True)
-    K = gf.LandauKernelsSU3(lat64)
+    K = kernels(lat64)
     for _ in range(args.sweeps):
         for parity in (0, 1):
             K.orStep(lat64.V // 2 // 32, 256, work, None, parity, 1.7)
     gf.synchronize()
     print("drop-in sweeps done", flush=True)
-    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gf.LANDAU, args.sweeps, 5, C.byref(ms),
+    _lib.call("gf_time_sweeps", C.byref(lat64.struct), C.c_void_p(work.ptr), gauge, args.sweeps, 5, C.byref(ms),
               None)
     print(f"drop-in quality: {ms.value:.4f} ms per kernel", flush=True)
 

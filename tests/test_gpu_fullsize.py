@@ -63,10 +63,12 @@ COULOMB_DIMS = (64, 32, 32, 32)
 
 def test_coulomb_32x32x32x64_every_slice(gf, gpu, oracle):
     """The Coulomb workload's lattice (32^3 x 64, sigma 0.3, seed 1, Haar
-    seed 2), every timeslice at once, in FP64 and in mixed precision: every
-    slice of both results has theta < 1e-12 by the oracle's per-slice quality,
-    the two agree slice by slice on the functional to 1e-10, and the mixed
-    result is a gauge transformation of the input (plaquette to 1e-12)."""
+    seed 2), every timeslice at once, in FP64 and in mixed precision -- the
+    bench's schedule (24-bit links, switch at 1e-12) and the workload's stated
+    one (FP32 12-real links until theta < 1e-6, then FP64): every slice of
+    each result has theta < 1e-12 by the oracle's per-slice quality, the
+    results agree slice by slice on the functional to 1e-10, and the mixed
+    results are gauge transformations of the input (plaquette to 1e-12)."""
     from app_schedule import to_timeslices
     from oracle_lib import COULOMB
 
@@ -76,24 +78,29 @@ def test_coulomb_32x32x32x64_every_slice(gf, gpu, oracle):
     U = gf.make_config(COULOMB_DIMS, 0.3, 1, 2)
     lat = gf.Lattice(COULOMB_DIMS)
     slices = {}
-    for name in ("fp64", "mixed"):
+    for name in ("fp64", "mixed", "mixed_fp32_1e-6"):
         d = gf.DeviceBuffer.from_numpy(U)
         if name == "fp64":
             r = gf.run_fast(lat, d, gf.COULOMB, 1.7, 1e-12, 20000, 100)
-        else:
+        elif name == "mixed":
             r = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-12, precision=1e-12, maxiter=20000)
+        else:
+            r = gf.run_mixed(lat, d, gf.COULOMB, 1.7, switch_precision=1e-6, precision=1e-12, maxiter=20000,
+                             mode=gf.MIXED_GAUGE32)
+            assert 0 < r.sweeps_fp32 < r.sweeps
         print(name, r, flush=True)
         assert r.converged
         out = d.download(np.float64, U.size)
         S = to_timeslices(out, COULOMB_DIMS)
         slices[name] = [oracle.quality(COULOMB, dims3, S[t].reshape(-1).copy()) for t in range(COULOMB_DIMS[0])]
         assert max(th for _, th in slices[name]) < 1e-12, name
-        if name == "mixed":
+        if name != "fp64":
             p_in = _plaquette(to_matrices(U, V), COULOMB_DIMS)
             p_out = _plaquette(to_matrices(out, V), COULOMB_DIMS)
             assert p_out == pytest.approx(p_in, rel=1e-12)
-    for (g64, _), (gm, _) in zip(slices["fp64"], slices["mixed"]):
-        assert gm == pytest.approx(g64, rel=1e-10)
+    for name in ("mixed", "mixed_fp32_1e-6"):
+        for (g64, _), (gm, _) in zip(slices["fp64"], slices[name]):
+            assert gm == pytest.approx(g64, rel=1e-10), name
 
 
 @pytest.mark.parametrize("L,copies", [(16, 4), (20, 4), (24, 2), (32, 1)])

@@ -575,3 +575,51 @@ def test_wavefront_sweeps_are_bit_identical(gf, gpu, gauge, dims):
                 assert np.array_equal(v[-1], want[k][-1]), (slab, k)
     finally:
         _set_wavefront(gf, prev)
+
+
+def _set_block_order(gf, tpass):
+    import ctypes as C
+    prev = C.c_int()
+    gf._lib.call("gf_set_block_order", int(tpass), C.byref(prev))
+    return prev.value
+
+
+@pytest.mark.parametrize("gauge,dims", [(LANDAU, (16, 8, 8, 8)), (LANDAU, (24, 4, 8, 8)), (COULOMB, (16, 4, 8, 8)),
+                                        (MAG, (16, 8, 8, 8))])
+def test_block_order_is_bit_identical(gf, gpu, gauge, dims):
+    """The XCD block order of the gauge-transformation sweep (each XCD's
+    timeslices walked in passes of 1, 2, 3 or all of them) changes which
+    block runs where and when, never a result: the mixed driver's run (FP32
+    phase with checks every 5 sweeps on 24-bit and FP32 links, FP32 phase in
+    FP64 arithmetic, and the FP64 phase on the FP64 gauge-transformation
+    sweep, all-FP64 mode) and the gauge-annealing schedule are bit-identical
+    for every order.  The shapes give each XCD 2 or 3 timeslices of whole
+    64-site blocks."""
+    U = _cfg(gf, dims, sigma=0.35 if gauge == MAG else 0.3)
+    lat = gf.Lattice(dims)
+    spx = dims[0] // 8
+    prev_wf = _set_wavefront(gf, 0)
+    prev = _set_block_order(gf, 0)
+    try:
+        want = None
+        for tpass in [0] + [p for p in range(1, spx) if spx % p == 0]:
+            _set_block_order(gf, tpass)
+            got = {}
+            for mode in (gf.MIXED_GAUGE24, gf.MIXED_GAUGE32, gf.MIXED_GAUGE64, gf.MIXED_FP64):
+                d = gf.DeviceBuffer.from_numpy(U)
+                r = gf.run_mixed(lat, d, gauge, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=23, mode=mode,
+                                 check32=5)
+                got[mode] = ((r.sweeps, r.checks, r.theta, r.gff), d.download())
+            if gauge != COULOMB:
+                d = gf.DeviceBuffer.from_numpy(U)
+                ctr = gf.run_sa(lat, d, gauge, 1.2, 0.1, 6, 2, 100, 3, 40, mode=2)
+                got["sa"] = ((ctr,), d.download())
+            if want is None:
+                want = got
+                continue
+            for k, v in got.items():
+                assert v[0] == want[k][0], (tpass, k)
+                assert np.array_equal(v[1], want[k][1]), (tpass, k)
+    finally:
+        _set_block_order(gf, prev)
+        _set_wavefront(gf, prev_wf)

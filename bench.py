@@ -488,8 +488,9 @@ def landau48_line(args) -> dict:
     m = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter, mode=args.mode,
                      refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
     print(f"bench: landau 48^4 mixed {m}", file=sys.stderr, flush=True)
-    slab = C.c_int()
+    slab, tpass = C.c_int(), C.c_int()
     _lib.call("gf_wavefront_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(slab))
+    _lib.call("gf_block_order_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(tpass))
     # the same run with every sweep as two kernels (no wavefront)
     prev = C.c_int()
     _lib.call("gf_set_wavefront", 0, C.byref(prev))
@@ -525,7 +526,8 @@ def landau48_line(args) -> dict:
                    "precision": args.precision, "switch_precision": args.switch, "mode": MODE_NAMES[args.mode]},
         "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
         "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,
-        "wavefront_slab": slab.value, "time_to_theta_mixed_two_kernel_sweeps_s": m2.ms_total / 1e3,
+        "wavefront_slab": slab.value, "block_order_tpass": tpass.value,
+        "time_to_theta_mixed_two_kernel_sweeps_s": m2.ms_total / 1e3,
         "mixed_identical_with_and_without_wavefront": (m.sweeps, m.gff, m.theta) == (m2.sweeps, m2.gff, m2.theta),
         "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
         "gff_fp64": f.gff, "gff_rel_diff_mixed_vs_fp64": abs(m.gff - f.gff) / abs(f.gff),
@@ -650,6 +652,14 @@ def main():
                                     "speedup_vs_fp64": None if fp64 is None else fp64.ms_total / rm.ms_total}
             print(f"bench: mode {MODE_NAMES[m]}: {rm}", file=sys.stderr, flush=True)
 
+    # the tuner's choices for this lattice (gauge-sweep modes)
+    tuning = None
+    if args.mode in (1, 2, 3):
+        slab, tpass = C.c_int(), C.c_int()
+        _lib.call("gf_wavefront_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(slab))
+        _lib.call("gf_block_order_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(tpass))
+        tuning = {"wavefront_slab": slab.value, "block_order_tpass": tpass.value}
+
     # kernel timing for the roofline (on the launch stream)
     U32 = gf.DeviceBuffer(lat32.nbytes)
 
@@ -748,6 +758,8 @@ def main():
         "gff_fp64": None if fp64 is None else fp64.gff,
         "gff_rel_diff_mixed_vs_fp64": None if fp64 is None else abs(r.gff - fp64.gff) / abs(fp64.gff),
         "speedup_mixed_vs_fp64": None if fp64 is None else fp64.ms_total / r.ms_total,
+        "speedup_mixed_vs_fastest_fp64": None if fp64 is None or fp64g is None
+        else min(fp64.ms_total, fp64g.ms_total) / r.ms_total,
         "fp64_gauge_sweep": None if fp64g is None else {
             "time_to_theta_s": fp64g.ms_total / 1e3, "sweeps": fp64g.sweeps, "gff": fp64g.gff,
             "theta": fp64g.theta, "gff_rel_diff_vs_fp64": abs(fp64g.gff - fp64.gff) / abs(fp64.gff),
@@ -761,6 +773,7 @@ def main():
             "theta": dropin.theta, "site_updates_per_s": dropin.sweeps * V / (dropin.ms_total / 1e3)},
         "site_updates_per_s_fp64": None if fp64 is None else fp64.sweeps * V / (fp64.ms_total / 1e3),
         "sweep_ms_in_run": {"fp32": t32, "fp64": t64},
+        "tuning": tuning,
         "kernel_ms": kms,
         "kernel_hbm_frac": {k: kbytes[k] / (kms[k] / 1e3) / 1e9 / HBM_PEAK_GBS for k in kms},
         "roofline": {

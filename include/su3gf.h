@@ -325,6 +325,10 @@ gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *s
  * a pass boundary.  The results do not depend on the order (the sites of a
  * parity are independent); the time does.  Chosen per lattice shape, gauge
  * and mode by timing, together with the wavefront slab.
+ * The drop-in quality kernel (gf_stats_quality, gf_quality_per_site) walks
+ * the XCDs' timeslices the same way, and its order is tuned on its first
+ * launch for a kind of lattice (timing launches read the links and write
+ * nothing), under the same setting.
  * gf_set_block_order: -1 choose by timing (default), 0 one pass, n > 0 passes
  * of n timeslices (where n divides the XCD's range; otherwise one pass);
  * *previous (optional) receives the old setting.

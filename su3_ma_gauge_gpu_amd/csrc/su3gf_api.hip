@@ -479,24 +479,38 @@ gf_status gf_stats_destroy(gf_stats *st)
     return GF_OK;
 }
 
+namespace {
+// the quality kernel's XCD block order for this lattice (QualityArgs::tpass),
+// tuned on first use (defined with the sweeps' tuner below)
+int quality_tpass(const gf_lattice *lat, int gauge, const void *U, hipStream_t s);
+
+hipError_t quality_kernel(const gf_lattice *lat, int gauge, const void *U, double *dGff, double *dA,
+                          double *partial, int crit_max, int tpass, hipStream_t s)
+{
+    int64_t V = 1;
+    for (int d = 0; d < 4; d++) V *= lat->dims[d];
+    // Coulomb on a 4D lattice: the spatial quality of every timeslice
+    const Geom g = make_geom(lat->dims, V);
+    if (lat->dtype == GF_F64) {
+        QualityArgs<double> a{};
+        a.U = (const double *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
+        a.tpass = tpass;
+        return exact::launch_quality<double>(lat->layout, gauge, a, s);
+    }
+    QualityArgs<float> a{};
+    a.U = (const float *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
+    a.tpass = tpass;
+    return exact::launch_quality<float>(lat->layout, gauge, a, s);
+}
+}  // namespace
+
 static gf_status quality_launch(const gf_lattice *lat, int gauge, const void *U, double *dGff, double *dA,
                                 double *partial, int crit_max, hipStream_t s)
 {
     int64_t V;
     GF_TRY(check_lattice(lat, &V));
     if (!U) return fail(GF_ERR_BAD_ARG, "U is null");
-    // Coulomb on a 4D lattice: the spatial quality of every timeslice
-    const Geom g = make_geom(lat->dims, V);
-    hipError_t e;
-    if (lat->dtype == GF_F64) {
-        QualityArgs<double> a{};
-        a.U = (const double *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
-        e = exact::launch_quality<double>(lat->layout, gauge, a, s);
-    } else {
-        QualityArgs<float> a{};
-        a.U = (const float *)U; a.g = g; a.dGff = dGff; a.dA = dA; a.partial = partial; a.crit_max = crit_max;
-        e = exact::launch_quality<float>(lat->layout, gauge, a, s);
-    }
+    const hipError_t e = quality_kernel(lat, gauge, U, dGff, dA, partial, crit_max, quality_tpass(lat, gauge, U, s), s);
     if (e != hipSuccess) return hip_fail(e, "quality kernel launch");
     return GF_OK;
 }
@@ -727,6 +741,56 @@ int wavefront_slab(GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
     }
     ga.tpass = g_tpass_setting >= 0 ? g_tpass_setting : tu.tpass;
     return g_slab_setting >= 0 ? g_slab_setting : tu.slab;
+}
+
+// The drop-in quality kernel walks each XCD's timeslices in the same kind of
+// passes (exact::launch_quality); it reads the links only, so candidates are
+// timed on the caller's buffer with no outputs (three launches each after one
+// unmeasured).  Same setting (gf_set_block_order) and 3% margin as the sweeps.
+std::map<std::array<int, 7>, int> g_quality_cache;
+
+int quality_tpass(const gf_lattice *lat, int gauge, const void *U, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (g_tpass_setting >= 0) return g_tpass_setting;
+    const std::array<int, 7> key{lat->dims[0], lat->dims[1], lat->dims[2], lat->dims[3], lat->dtype, lat->layout,
+                                 gauge};
+    const auto hit = g_quality_cache.find(key);
+    if (hit != g_quality_cache.end()) return hit->second;
+    int64_t V = 1;
+    for (int d = 0; d < 4; d++) V *= lat->dims[d];
+    const int64_t half_slice = V / 2 / lat->dims[0];
+    const int spx = lat->dims[0] % 8 == 0 && half_slice % 128 == 0 && exact::quality_blocks(V) % 8 == 0
+                        ? lat->dims[0] / 8 : 0;
+    int best = 0;
+    EventPair ev;
+    if (spx > 1 && hipEventCreate(&ev.a) == hipSuccess && hipEventCreate(&ev.b) == hipSuccess) {
+        auto time = [&](int P) -> float {
+            if (quality_kernel(lat, gauge, U, nullptr, nullptr, nullptr, 0, P, s) != hipSuccess) return -1.f;
+            if (hipEventRecord(ev.a, s) != hipSuccess) return -1.f;
+            for (int k = 0; k < 3; k++)
+                if (quality_kernel(lat, gauge, U, nullptr, nullptr, nullptr, 0, P, s) != hipSuccess) return -1.f;
+            float ms = 0;
+            if (hipEventRecord(ev.b, s) != hipSuccess || hipEventSynchronize(ev.b) != hipSuccess ||
+                hipEventElapsedTime(&ms, ev.a, ev.b) != hipSuccess)
+                return -1.f;
+            return ms;
+        };
+        const float one_pass = time(0);
+        float best_ms = one_pass;
+        for (int P = 1; P < spx && one_pass > 0; P++) {
+            if (spx % P != 0) continue;
+            const float ms = time(P);
+            if (ms > 0 && ms < best_ms && ms < 0.97f * one_pass) {
+                best_ms = ms;
+                best = P;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipGetLastError();
+    g_quality_cache[key] = best;
+    return best;
 }
 }  // namespace
 

@@ -264,8 +264,11 @@ template <typename R> __device__ __forceinline__ int64_t quality_block(const Qua
         const int64_t L = (int64_t)xcd * per + k;
         return L < a.nblocks ? L : -1;
     }
-    const uint32_t q = fdiv(k, a.spx_div);
-    const int64_t t = (int64_t)xcd * a.slices_per_xcd + (k - q * (uint32_t)a.slices_per_xcd);
+    const uint32_t P = (uint32_t)a.tpass;
+    const uint32_t pass = fdiv(k, a.pass_div);
+    const uint32_t rem = k - pass * P * (uint32_t)a.slice_blocks;
+    const uint32_t q = fdiv(rem, a.tpass_div);
+    const int64_t t = (int64_t)xcd * a.slices_per_xcd + pass * P + (rem - q * P);
     return t * a.slice_blocks + q;
 }
 
@@ -326,7 +329,9 @@ static hipError_t launch_quality_s(int gauge, QualityArgs<R> a, hipStream_t s)
         a.remap = 2;
         a.slice_blocks = half_slice / kQualitySites;
         a.slices_per_xcd = a.g.N[0] / 8;
-        a.spx_div = make_fastdiv((uint32_t)a.slices_per_xcd);
+        if (a.tpass <= 0 || a.slices_per_xcd % a.tpass != 0) a.tpass = a.slices_per_xcd;
+        a.tpass_div = make_fastdiv((uint32_t)a.tpass);
+        a.pass_div = make_fastdiv((uint32_t)(a.tpass * a.slice_blocks));
         grid = (unsigned)a.nblocks;
     } else {
         a.remap = 1;

@@ -73,7 +73,8 @@ template <typename R> struct QualityArgs {
     int64_t nblocks;        // logical blocks (quality_blocks)
     int64_t slice_blocks;   // remap 2: logical blocks per timeslice
     int slices_per_xcd;     // remap 2
-    FastDiv spx_div;        // division by slices_per_xcd
+    int tpass;              // remap 2: each XCD's timeslices walked in passes of tpass (0: one pass)
+    FastDiv tpass_div, pass_div;  // division by tpass and by tpass * slice_blocks
 };
 
 

@@ -250,6 +250,42 @@ def test_quality(gf, gpu, oracle, gauge):
     assert np.allclose(da.download(), ao, rtol=1e-11, atol=1e-13)
 
 
+@pytest.mark.parametrize("gauge", [LANDAU, MAG])
+def test_quality_block_order(gf, gpu, oracle, gauge):
+    """The drop-in quality kernel on a lattice whose XCDs each take 2 and 3
+    timeslices, in every XCD block order (one pass, passes of 1) and in the
+    order its tuner picks: per-site values, the reduced gff and theta (AVERAGE
+    and MAX) identical bit for bit across orders, and the oracle's."""
+    import ctypes as C
+    for dims in ((16, 4, 8, 8), (24, 4, 4, 16)):
+        V = int(np.prod(dims))
+        U = _config(gf, dims)
+        d = gf.DeviceBuffer.from_numpy(U)
+        lat = gf.Lattice(dims)
+        K = getattr(gf, KERNELS[gauge])
+        prev = C.c_int()
+        gf._lib.call("gf_set_block_order", 0, C.byref(prev))
+        got = []
+        try:
+            for tpass in (0, 1, -1):
+                gf._lib.call("gf_set_block_order", tpass, None)
+                st = gf.GaugeFixingStats(K, d, lat)
+                st.generateGaugeQuality()
+                st_max = gf.GaugeFixingStats(K, d, lat, crit=gf.CRIT_MAX)
+                st_max.generateGaugeQuality()
+                dg, da = gf.DeviceBuffer(8 * V), gf.DeviceBuffer(8 * V)
+                K(lat).generateGaugeQualityPerSite(V // 32, 32, d, dg, da)
+                got.append((st.getCurrentGff(), st.getCurrentA(), st_max.getCurrentA(), dg.download(), da.download()))
+        finally:
+            gf._lib.call("gf_set_block_order", prev.value, None)
+        for g in got[1:]:
+            assert g[:3] == got[0][:3], dims
+            assert np.array_equal(g[3], got[0][3]) and np.array_equal(g[4], got[0][4]), dims
+        g_ref, t_ref = oracle.quality(gauge, dims, U)
+        assert got[0][0] == pytest.approx(g_ref, rel=1e-13)
+        assert got[0][1] == pytest.approx(t_ref, rel=1e-12)
+
+
 def test_quality_coulomb_and_max(gf, gpu, oracle):
     dims3 = (1, 8, 6, 4)
     U = _config(gf, dims3, seed=3, haar=4)

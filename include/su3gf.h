@@ -309,13 +309,30 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  * two-kernel sweep: results are bit-identical either way.  The slab is
  * chosen per lattice shape, gauge and mode by timing a few sweeps with each
  * candidate on the first run of that kind in the process.
+ * GF_WAVEFRONT_PIPELINED stands for the pipelined sweep, one launch per
+ * sweep: each XCD takes Nt/8 timeslices, and the odd update of a block of 64
+ * sites follows the even updates of its neighbours within the same launch
+ * (per-block completion flags), so that its links and neighbours come from
+ * the XCD's L2.  Landau and U1xU1, Nt a multiple of 8, a timeslice's sites of
+ * a parity a whole number of 64-site blocks; bit-identical results as well.
  * gf_set_wavefront: -1 choose by timing (default), 0 never, n > 0 slabs of
- * n timeslices; *previous (optional) receives the old setting.
+ * n timeslices (or GF_WAVEFRONT_PIPELINED); *previous (optional) receives the
+ * old setting.
  * gf_wavefront_choice: the slab in use for gf_run_mixed of lat with `gauge`
  * and gauge-sweep `mode` (0 two kernels, -1 not chosen yet).
  * No reference counterpart. */
+#define GF_WAVEFRONT_PIPELINED 1000000
 gf_status gf_set_wavefront(int slab, int *previous);
 gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int mode, int *slab);
+/* Pipelined sweeps (GF_WAVEFRONT_PIPELINED) this process has launched, so a
+ * caller can see which path its sweeps took.  gf_set_pipelined_lag: the extra
+ * even blocks each XCD updates before it starts the odd block they are ahead
+ * of (-1 chosen by the timing with the slab, default; n >= 0 fixed): more
+ * lets the even updates an odd block waits for finish first, less keeps that
+ * block's links in the L2.  Results do not depend on it.  No reference
+ * counterpart. */
+gf_status gf_pipelined_launches(long long *n);
+gf_status gf_set_pipelined_lag(int lag, int *previous);
 
 /* The gauge-transformation sweep's block order across the eight XCDs (each
  * with its own L2): each XCD takes a contiguous range of timeslices and walks

@@ -48,6 +48,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     U1XU1,
     UPDATE_DP,
     SA_GRAPH,
+    WAVEFRONT_PIPELINED,
     FILE_PLAIN,
     FILE_HEADERONLY,
     FILE_VOGT,

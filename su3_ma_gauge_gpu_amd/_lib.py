@@ -18,6 +18,7 @@ SA_GRAPH = 0x10  # OR'ed into an annealing mode: the schedule captured into one 
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
 MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24, MIXED_FP64 = 0, 1, 2, 3, 4
+WAVEFRONT_PIPELINED = 1000000  # gf_set_wavefront: the pipelined sweep, one launch per sweep
 CRIT_MAX, CRIT_AVERAGE = 0, 1
 FILE_PLAIN, FILE_HEADERONLY, FILE_VOGT, FILE_ILDG, FILE_QCDSTAG = range(5)
 REAL_STANDARD, REAL_DOUBLE, REAL_FLOAT = range(3)
@@ -104,6 +105,8 @@ SIGNATURES = {
     "gf_time_sweeps": (_i, [_pl, _vp, _i, _i, _i, C.POINTER(_f), _vp]),
     "gf_set_wavefront": (_i, [_i, C.POINTER(_i)]),
     "gf_wavefront_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
+    "gf_pipelined_launches": (_i, [C.POINTER(C.c_longlong)]),
+    "gf_set_pipelined_lag": (_i, [_i, C.POINTER(_i)]),
     "gf_set_block_order": (_i, [_i, C.POINTER(_i)]),
     "gf_block_order_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),

@@ -141,6 +141,32 @@ struct Scratch {
     template <typename T> T *as() const { return (T *)p; }
 };
 
+// The synchronisation words of a lattice's pipelined sweeps
+// (launch_gsweep_pipe), allocated and zeroed only where that sweep applies.
+// check() reports a dependency wait that ran out of its spin limit.
+struct PipeBuf {
+    Scratch mem;
+    uint32_t count = 0;  // launches on these words (block-index order)
+    hipError_t init(const Geom &geo, int gauge, bool links64, hipStream_t s)
+    {
+        if (!gsweep_pipe_applies(geo, gauge, links64)) return hipSuccess;
+        const size_t bytes = sizeof(uint32_t) * pipe_sync_words(sweep_grid(geo));
+        const hipError_t e = mem.alloc(bytes, s);
+        return e != hipSuccess ? e : hipMemsetAsync(mem.p, 0, bytes, s);
+    }
+    PipeSync *get() const { return mem.as<PipeSync>(); }
+    gf_status check(hipStream_t s) const
+    {
+        if (!mem.p) return GF_OK;
+        uint32_t err = 0;
+        if (hipMemcpyAsync(&err, &get()->err, sizeof err, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail(GF_ERR_HIP, "reading the pipelined sweep's state");
+        if (err) return fail(GF_ERR_HIP, "a pipelined sweep's dependency wait ran out of its spin limit");
+        return GF_OK;
+    }
+};
+
 // Pinned host copies of the drivers' device state, one per host thread.  They
 // are never freed (freeing pinned memory would synchronise the device): a
 // thread that ends puts its copy on a process-wide free list, where the next
@@ -628,10 +654,14 @@ std::mutex g_slab_mu;
 struct GsweepTuning {
     int slab = 0;
     int tpass = 0;
+    int pipe_slack = 0;  // the pipelined sweep's extra lag (slab == kPipelinedSlab)
 };
 std::map<std::array<int, 8>, GsweepTuning> g_slab_cache;
 int g_slab_setting = -1;   // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
 int g_tpass_setting = -1;  // -1 tuned, 0 one pass, n > 0 passes of n timeslices (gf_set_block_order)
+int g_lag_setting = -1;    // -1 tuned, n >= 0 the pipelined sweep's extra lag (gf_set_pipelined_lag)
+// extra lags the tuner tries for the pipelined sweep
+constexpr int kPipeLags[] = {192, 256, 320, 384};
 
 // links: 0 FP32, 1 24-bit, 2 FP64 (with an FP64 g)
 std::array<int, 8> slab_key(const Geom &geo, int gauge, int links, bool fp64)
@@ -716,6 +746,23 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
                 tu.slab = c;
             }
         }
+        // the pipelined sweep, on synchronisation words of its own, for each extra lag
+        PipeBuf pb;
+        if (two > 0 && pb.init(geo, gauge, ga.links64 != 0, s) == hipSuccess && pb.get()) {
+            t.pipe = pb.get();
+            t.pipe_count = &pb.count;
+            for (int lag : kPipeLags) {
+                if (g_lag_setting >= 0 && lag != kPipeLags[0]) break;
+                t.pipe_slack = g_lag_setting >= 0 ? g_lag_setting : lag;
+                const float ms = time_gsweep(t, gauge, fp64, kPipelinedSlab, 3, s, ev.a, ev.b);
+                if (ms > 0 && ms < best_ms && ms < 0.97f * two && pb.check(s) == GF_OK) {
+                    best_ms = ms;
+                    tu.slab = kPipelinedSlab;
+                    tu.pipe_slack = t.pipe_slack;
+                }
+            }
+            t.pipe = nullptr;
+        }
     }
     (void)hipStreamSynchronize(s);
     (void)hipGetLastError();
@@ -740,6 +787,7 @@ int wavefront_slab(GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
         g_slab_cache[key] = tu;
     }
     ga.tpass = g_tpass_setting >= 0 ? g_tpass_setting : tu.tpass;
+    ga.pipe_slack = g_lag_setting >= 0 ? g_lag_setting : tu.pipe_slack;
     return g_slab_setting >= 0 ? g_slab_setting : tu.slab;
 }
 
@@ -837,6 +885,22 @@ extern "C" gf_status gf_wavefront_choice(const gf_lattice *lat, int gauge, int m
     return GF_OK;
 }
 
+extern "C" gf_status gf_set_pipelined_lag(int lag, int *previous)
+{
+    if (lag < -1) return fail(GF_ERR_BAD_ARG, "pipelined lag setting %d (-1 tuned, n >= 0 fixed)", lag);
+    std::lock_guard<std::mutex> lock(g_slab_mu);
+    if (previous) *previous = g_lag_setting;
+    g_lag_setting = lag;
+    return GF_OK;
+}
+
+extern "C" gf_status gf_pipelined_launches(long long *n)
+{
+    if (!n) return fail(GF_ERR_BAD_ARG, "n is null");
+    *n = pipe_launches();
+    return GF_OK;
+}
+
 extern "C" gf_status gf_block_order_choice(const gf_lattice *lat, int gauge, int mode, int *tpass)
 {
     if (!tpass) return fail(GF_ERR_BAD_ARG, "tpass is null");
@@ -875,6 +939,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
     l32.dtype = GF_F32;
     l32.layout = GF_LAYOUT_R12;
     Scratch bU32, bg;
+    PipeBuf pipe;
     GaugeSweepArgs ga{};
     const Geom geo = make_geom(lat->dims, V);
     if (gform) {
@@ -890,6 +955,11 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
         ga.remap = 1;
         ga.seed = (uint32_t)seed;
         ga.nsb = 32;
+        // (a captured schedule keeps the two-kernel sweeps: the pipelined
+        // sweep's launch numbers come from the host at each launch)
+        if (!graph) HIP_TRY(pipe.init(geo, gauge, false, s));
+        ga.pipe = pipe.get();
+        ga.pipe_count = &pipe.count;
     }
     auto project = [&]() -> gf_status {
         if (gform) return GF_OK;  // the gauge-transformation sweep keeps g unitary itself
@@ -1003,6 +1073,7 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
                                    s));
         GF_TRY(gf_project(lat, U, s));
         HIP_TRY(hipStreamSynchronize(s));
+        GF_TRY(pipe.check(s));
     }
     if (temperatures) *temperatures = n;
     return GF_OK;
@@ -1403,8 +1474,12 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     const Geom geo = make_geom(lat->dims, V);
     int slab = 0;
     GaugeSweepArgs wa{};  // variant 6: whole sweeps as the mixed run's FP32 phase runs them
+    PipeBuf pipe;
     if (variant == 6) {
         wa.U0 = pbuf.as<const float>(); wa.g = gb.p; wa.geo = geo; wa.param = 1.7f; wa.remap = 1; wa.links24 = 1;
+        HIP_TRY(pipe.init(geo, gauge, false, s));
+        wa.pipe = pipe.get();
+        wa.pipe_count = &pipe.count;
         slab = wavefront_slab(wa, gauge, false, s);
     }
     int tpass = 0;  // the gauge-transformation sweeps' block order, as the drivers use it
@@ -1471,6 +1546,7 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
     HIP_TRY(hipEventSynchronize(ev.b));
     HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
     *ms_per_kernel = ms / (2.0f * nsweeps);
+    GF_TRY(pipe.check(s));
     return GF_OK;
 }
 
@@ -1565,6 +1641,10 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         ga.param = omega;
         ga.remap = 1;
         ga.links24 = mode == GF_MIXED_GAUGE24;
+        PipeBuf pipe;
+        HIP_TRY(pipe.init(geo, gauge, false, s));
+        ga.pipe = pipe.get();
+        ga.pipe_count = &pipe.count;
         auto launch = [&](int parity, int theta, double *partial, int) {
             ga.parity = parity;
             ga.partial = partial;
@@ -1600,6 +1680,7 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
                               kStall32, false, false, sweep_grid(geo), fc, s, launch64, none, plain64, &done_b, out));
             done32 += done_b;
         }
+        GF_TRY(pipe.check(s));
     }
     out->sweeps_fp32 = done32;
     const bool reached_switch = fp32_phase && out->converged == 1;

@@ -39,6 +39,7 @@
 // Re tr of every link (each link has exactly one endpoint of either parity).
 // These are the quantities of LandauKernelsSU3.hxx:114-191.
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "su3gf_sweep.h"
@@ -138,8 +139,13 @@ __device__ __forceinline__ uint32_t tile_off(uint32_t boff, uint32_t words)
     return (boff & ~255u) * words + (boff & 255u);
 }
 
+// Cache policy of a buffer access (the instruction's sc1 bit): coherent at
+// device scope, for g that a block on another XCD writes or reads within the
+// same launch (the pipelined sweep); 0 is the default policy.
+constexpr int kSc1 = 16;
+
 // rows 0,1 of site byte offset boff
-template <typename A> __device__ __forceinline__ void load_rows01(const Field &f, uint32_t boff, M3<A> &u)
+template <typename A, int CP = 0> __device__ __forceinline__ void load_rows01(const Field &f, uint32_t boff, M3<A> &u)
 {
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -147,9 +153,9 @@ template <typename A> __device__ __forceinline__ void load_rows01(const Field &f
         for (int j = 0; j < 3; j++) {
             const int comp = 2 * (j + 3 * i);
             const float re =
-                __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)(comp * f.V4), 0));
+                __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)(comp * f.V4), CP));
             const float im = __builtin_bit_cast(
-                float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((comp + 1) * f.V4), 0));
+                float, __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)((comp + 1) * f.V4), CP));
             u.e[i][j] = cx((A)re, (A)im);
         }
 }
@@ -246,10 +252,11 @@ template <int LF> __device__ __forceinline__ Field g_field(const GaugeSweepArgs 
     if (LF == 2) return make_tiled_field64((const double *)a.g, a.geo.V);
     return make_tiled_field(a.g, a.geo.V, 12);
 }
-template <typename A, int LF> __device__ __forceinline__ void load_g(const Field &f, uint32_t boff, M3<A> &u)
+template <typename A, int LF, int CP = 0> __device__ __forceinline__ void load_g(const Field &f, uint32_t boff, M3<A> &u)
 {
+    static_assert(LF != 2 || CP == 0, "FP64 g is only swept by the two kernels");
     if (LF == 2) load_rows01_64(f, tile_off64(boff), u);
-    else load_rows01(f, tile_off(boff, 12), u);
+    else load_rows01<A, CP>(f, tile_off(boff, 12), u);
 }
 
 typedef unsigned int B64 __attribute__((__vector_size__(8)));  // the 64-bit buffer store's operand
@@ -268,7 +275,7 @@ template <typename A> __device__ __forceinline__ void store_rows01_64(const Fiel
         }
 }
 
-template <typename A> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
+template <typename A, int CP = 0> __device__ __forceinline__ void store_rows01(const Field &f, uint32_t boff, const M3<A> &u)
 {
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -276,16 +283,17 @@ template <typename A> __device__ __forceinline__ void store_rows01(const Field &
         for (int j = 0; j < 3; j++) {
             const int comp = 2 * (j + 3 * i);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)u.e[i][j].x), f.r, (int)boff,
-                                                  (int)(comp * f.V4), 0);
+                                                  (int)(comp * f.V4), CP);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, (float)u.e[i][j].y), f.r, (int)boff,
-                                                  (int)((comp + 1) * f.V4), 0);
+                                                  (int)((comp + 1) * f.V4), CP);
         }
 }
 
-template <typename A, int LF> __device__ __forceinline__ void store_g(const Field &f, uint32_t boff, const M3<A> &u)
+template <typename A, int LF, int CP = 0> __device__ __forceinline__ void store_g(const Field &f, uint32_t boff, const M3<A> &u)
 {
+    static_assert(LF != 2 || CP == 0, "FP64 g is only swept by the two kernels");
     if (LF == 2) store_rows01_64(f, tile_off64(boff), u);
-    else store_rows01(f, tile_off(boff, 12), u);
+    else store_rows01<A, CP>(f, tile_off(boff, 12), u);
 }
 
 // Z = rows 0,1 of X Y^dagger (Y complete), then row 2 of the SU(3) product
@@ -510,20 +518,23 @@ __device__ __forceinline__ void gauge_update(A q[4], float param, Philox &rng)
 
 // The direction-mu terms of site x (coordinates c, parity par):
 // X = U0_mu(x) g(y)^dagger and Z = g(z) U0_mu(z), y = x+mu, z = x-mu, so that
-// S_mu = X^dagger + Z.
+// S_mu = X^dagger + Z.  coh (uniform): the neighbours' g through loads
+// coherent at device scope.
 template <typename A, int LF>
 __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const Field &fg, int mu, const int c[4],
-                                                int par, uint32_t site_off, M3<A> &X, M3<A> &Z)
+                                                int par, uint32_t site_off, M3<A> &X, M3<A> &Z, bool coh = false)
 {
     M3<A> T, W;
     const uint32_t y_off = site_boff<LF>(forward(a.geo, c, par, mu));
     const uint32_t z_off = site_boff<LF>(backward(a.geo, c, par, mu));
     const Field fu = links_field<LF>(a.U0, a.geo.V, mu);
     load_links<A, LF>(fu, site_off, W);
-    load_g<A, LF>(fg, y_off, T);
+    if (LF != 2 && coh) load_g<A, LF, LF == 2 ? 0 : kSc1>(fg, y_off, T);
+    else load_g<A, LF>(fg, y_off, T);
     third_row(T);
     prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
-    load_g<A, LF>(fg, z_off, T);
+    if (LF != 2 && coh) load_g<A, LF, LF == 2 ? 0 : kSc1>(fg, z_off, T);
+    else load_g<A, LF>(fg, z_off, T);
     load_links<A, LF>(fu, z_off, W);
     third_row(W);
     prod_ab(Z, T, W);  // g(z) U0_mu(z)
@@ -535,7 +546,8 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
 // partial sums of the 64-site block blk.
 template <typename A, int G, int ALGO, int THETA, int LF, typename SumS>
 __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &fg, uint32_t site_off, bool active,
-                                          int64_t blk, int lane, const SumS &sum_s, M3<A> Gx, Philox &rng)
+                                          int64_t blk, int lane, const SumS &sum_s, M3<A> Gx, Philox &rng,
+                                          bool sc1_store = false)
 {
     static_assert(THETA == 0 || ALGO != GF_RANDOM, "the random transformation forms no S to check");
     third_row(Gx);
@@ -588,7 +600,10 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
     }
 
     project_rows01(Gx);  // keep g unitary at the precision it is stored in
-    if (active) store_g<A, LF>(fg, site_off, Gx);
+    if (active) {
+        if (LF != 2 && sc1_store) store_g<A, LF, LF == 2 ? 0 : kSc1>(fg, site_off, Gx);
+        else store_g<A, LF>(fg, site_off, Gx);
+    }
 
     if (THETA == 2 && active) {  // the state after the update, with g(x) as stored
         third_row(Gx);
@@ -690,6 +705,214 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
             }
     };
     site_tail<A, G, ALGO, THETA, LF>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng);
+}
+
+// ------------------------------------------------------ pipelined sweep
+// One launch per sweep (su3gf_pipe.h): block b serves queue b % 8 (the XCD it
+// runs on) with the item of ticket b / 8 -- the even or the odd sites of one
+// 64-site block.  An odd item waits for the flags of the even blocks around it
+// (slices t-1..t+1, blocks q-R..q+R), then runs as the two-kernel sweep's odd
+// kernel would; an even item runs as the even kernel and then sets its flag to
+// the launch's number.  Every site thus sees the neighbour values of the
+// two-kernel sweep, and the results are bit-identical.  An item waits only for
+// items of lower block index, which the hardware has dispatched before it.
+//
+// Memory.  An even item writes its g with stores coherent at device scope
+// (sc1), waits for them and then sets its flag, also at device scope.  An odd
+// item loads its links and its own g (not written in the launch) before it
+// waits, and the neighbours' g after seeing the flags: through the XCD's L2
+// (where the even items of its own queue, on the same XCD, wrote it) with
+// plain loads, and with device-scope loads where the writer may have run on
+// another XCD -- the time-direction neighbour in the next queue's or the
+// previous queue's timeslice, and any block whose flag says it ran elsewhere
+// (never seen on this hardware, where block b runs on XCD b % 8, but handled).
+
+// the XCD this wave runs on (hardware register XCC_ID, bits 3:0)
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return __builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) & 7u;
+}
+
+// iterations (about a microsecond each) one dependency wait spins before it
+// gives up and sets the error word
+constexpr uint32_t kPipeSpin = 1u << 20;
+
+__device__ __forceinline__ uint32_t *pipe_flags(PipeSync *ps) { return (uint32_t *)(ps + 1); }
+
+// A link's stored words (12 FP32 values, or nine words of 24-bit values),
+// loaded ahead of the arithmetic; decode_link gives what load_links gives.
+struct LinkWords {
+    uint32_t w[12];
+};
+template <int LF> __device__ __forceinline__ void load_link_words(const Field &f, uint32_t boff, LinkWords &l)
+{
+    static_assert(LF != 2, "FP32 or 24-bit links");
+    if (LF == 1) {
+        const uint32_t o = tile_off(boff, 9);
+#pragma unroll
+        for (int k = 0; k < 9; k++) l.w[k] = __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)o, (int)(k * f.V4), 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) l.w[k] = __builtin_amdgcn_raw_buffer_load_b32(f.r, (int)boff, (int)(k * f.V4), 0);
+    }
+}
+template <typename A, int LF> __device__ __forceinline__ void decode_link(const LinkWords &l, M3<A> &u)
+{
+    A v[12];
+    if (LF == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {  // as load_rows01_q24
+            const uint32_t w0 = l.w[3 * k], w1 = l.w[3 * k + 1], w2 = l.w[3 * k + 2];
+            const int a = (int)__builtin_amdgcn_sbfe((int)w0, 0, 24);
+            const int b = (int)__builtin_amdgcn_sbfe((int)__builtin_amdgcn_alignbit(w1, w0, 24), 0, 24);
+            const int c = (int)__builtin_amdgcn_sbfe((int)__builtin_amdgcn_alignbit(w2, w1, 16), 0, 24);
+            const int d = (int)w2 >> 8;
+            v[4 * k + 0] = (A)a * (A)kInvQ24;
+            v[4 * k + 1] = (A)b * (A)kInvQ24;
+            v[4 * k + 2] = (A)c * (A)kInvQ24;
+            v[4 * k + 3] = (A)d * (A)kInvQ24;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) v[k] = (A)__builtin_bit_cast(float, l.w[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) u.e[i][j] = cx(v[2 * (j + 3 * i)], v[2 * (j + 3 * i) + 1]);
+}
+
+template <typename A, int G, int ALGO, int LF>
+__global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
+{
+    static_assert(G == GF_LANDAU || G == GF_U1XU1, "four directions in S");
+    static_assert(LF != 2, "FP32 or 24-bit links");
+    constexpr int kW = 4;
+    __shared__ A lds[18 * kW * 64];
+    if (a.state && a.state->done) return;  // converged: uniform early exit
+    PipeSync *const ps = a.pipe;
+    const PipePlan &pl = a.plan;
+    const int queue = blockIdx.x & (kPipeQueues - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mu = wave;
+    const int lane = threadIdx.x & 63;
+    const uint32_t launch = a.pipe_launch;  // even flags of this launch hold it
+    int odd;
+    int64_t idx;
+    pipe_item(pl, blockIdx.x >> 3, odd, idx);
+    const uint32_t qb = fdiv((uint32_t)idx, a.pipe_tdiv);
+    const int t0 = queue * pl.T;
+    const int t = t0 + (int)((uint32_t)idx - qb * (uint32_t)pl.T);
+    const int64_t blk = (int64_t)t * pl.Q + qb;
+    const bool moved = xcc_id() != (uint32_t)queue;
+    const bool edge = t == t0 || t == t0 + pl.T - 1;
+
+    // what the item reads that the launch does not write: its links and its own g
+    bool active;
+    const int64_t p = block_site(a.geo, blk, lane, active);
+    const int64_t site = p + (odd ? a.geo.Vh : 0);
+    int c[4], par;
+    site_coords(a.geo, site, c, par);
+    const uint32_t site_off = site_boff<LF>(site);
+    const uint32_t y_off = site_boff<LF>(forward(a.geo, c, par, mu));
+    const uint32_t z_off = site_boff<LF>(backward(a.geo, c, par, mu));
+    const Field fg = g_field<LF>(a);
+    const Field fu = links_field<LF>(a.U0, a.geo.V, mu);
+    LinkWords lx, lz;
+    load_link_words<LF>(fu, site_off, lx);  // U0_mu(x)
+    load_link_words<LF>(fu, z_off, lz);     // U0_mu(z)
+    M3<A> Gx;
+    if (wave == 0) load_g<A, LF>(fg, site_off, Gx);
+
+    bool coh = moved;
+    if (odd) {
+        const int span = 2 * pl.R + 1;
+        int seen_moved = 0;
+        for (int j = threadIdx.x; j < 3 * span; j += kGBlock) {
+            const int dtj = j / span;
+            int tt = t + dtj - 1;
+            if (tt < 0) tt += a.geo.N[0];
+            else if (tt >= a.geo.N[0]) tt -= a.geo.N[0];
+            int64_t qq = (int64_t)qb + (j - dtj * span) - pl.R;
+            if (qq < 0) qq += pl.Q;
+            else if (qq >= pl.Q) qq -= pl.Q;
+            const uint32_t *f = pipe_flags(ps) + (int64_t)tt * pl.Q + qq;
+            for (uint32_t it = 0;; it++) {
+                const uint32_t v = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & ~kPipeMovedBit) == launch) {
+                    seen_moved |= (v & kPipeMovedBit) != 0;
+                    break;
+                }
+                if ((it & 63) == 63) {
+                    if (it >= kPipeSpin) {
+                        __hip_atomic_store(&ps->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    if (__hip_atomic_load(&ps->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        coh = __syncthreads_or(seen_moved) || moved;
+    }
+
+    Philox rng;
+    if (ALGO == GF_SA) {  // the key of the site's (mu = 0, up) reference thread; odd sites: counter + 1
+        const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
+        philox_init(rng, tid, a.seed, odd ? a.counter + 1 : a.counter);
+    }
+    {
+        // the direction-mu terms as direction_terms forms them; the neighbours
+        // of an odd site are even sites written in this launch
+        const bool coh_g = odd && (coh || (mu == 0 && edge));
+        M3<A> X, Z, T, W;
+        decode_link<A, LF>(lx, W);
+        if (coh_g) load_g<A, LF, kSc1>(fg, y_off, T);
+        else load_g<A, LF>(fg, y_off, T);
+        third_row(T);
+        prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
+        if (coh_g) load_g<A, LF, kSc1>(fg, z_off, T);
+        else load_g<A, LF>(fg, z_off, T);
+        decode_link<A, LF>(lz, W);
+        third_row(W);
+        prod_ab(Z, T, W);  // g(z) U0_mu(z)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 2 * (3 * i + j);
+                lds[((k + 0) * kW + wave) * 64 + lane] = X.e[j][i].x + Z.e[i][j].x;
+                lds[((k + 1) * kW + wave) * 64 + lane] = Z.e[i][j].y - X.e[j][i].y;
+            }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    const auto sum_s = [&](M3<A> &S) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 2 * (3 * i + j);
+                A v[2];
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const A *b = lds + (k + r) * kW * 64 + lane;
+                    v[r] = ((b[0] + b[64]) + b[128]) + b[192];
+                }
+                S.e[i][j] = cx(v[0], v[1]);
+            }
+    };
+    // even sites: g stored coherently at device scope (read by odd items of this launch)
+    site_tail<A, G, ALGO, 0, LF>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng, !odd);
+    if (odd) return;
+    // the stores are complete before the flag says so
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    if (lane == 0) {
+        __hip_atomic_store(pipe_flags(ps) + blk, launch | (moved ? kPipeMovedBit : 0u), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        if (moved) atomicAdd(&ps->moved, 1u);
+    }
 }
 
 // MAG on the gauge transformation.  MAG's subgroup quaternion is quadratic in
@@ -1043,6 +1266,7 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
 hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, int slab,
                                    hipStream_t s)
 {
+    if (slab == kPipelinedSlab) return launch_gsweep_pipe(gauge, algo, fp64_arith, a, s);
     if (!gauge_sweep_fits(a.geo.V)) return hipErrorInvalidValue;
     if (a.links64 && (!fp64_arith || !gauge_sweep64_fits(a.geo.V))) return hipErrorInvalidValue;
     const int N = a.geo.N[0];
@@ -1081,6 +1305,61 @@ hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const G
         if (err != hipSuccess) return err;
     }
     return hipSuccess;
+}
+
+namespace {
+template <typename A, int G, int ALGO, int LF> hipError_t launch_k_pipe(const GaugeSweepArgs &a, hipStream_t s)
+{
+    const dim3 grid((unsigned)(kPipeQueues * 2 * a.plan.n)), block(kGBlock);
+    hipLaunchKernelGGL((k_gsweep_pipe<A, G, ALGO, LF>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+template <int G, int LF> hipError_t launch_pipe_algo(int algo, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
+{
+    if (fp64_arith) return algo == GF_OR ? launch_k_pipe<double, G, GF_OR, LF>(a, s) : hipErrorNotSupported;
+    if (algo == GF_OR) return launch_k_pipe<float, G, GF_OR, LF>(a, s);
+    if (algo == GF_MICRO) return launch_k_pipe<float, G, GF_MICRO, LF>(a, s);
+    if (algo == GF_SA) return launch_k_pipe<float, G, GF_SA, LF>(a, s);
+    return hipErrorNotSupported;
+}
+template <int G> hipError_t launch_pipe_lf(int algo, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
+{
+    return a.links24 ? launch_pipe_algo<G, 1>(algo, fp64_arith, a, s) : launch_pipe_algo<G, 0>(algo, fp64_arith, a, s);
+}
+}  // namespace
+
+namespace {
+std::atomic<long long> g_pipe_launches{0};
+}
+long long pipe_launches() { return g_pipe_launches.load(); }
+
+bool gsweep_pipe_applies(const Geom &geo, int gauge, bool links64)
+{
+    if (links64 || geo.slice_bps != 0 || (gauge != GF_LANDAU && gauge != GF_U1XU1)) return false;
+    if (!gauge_sweep_fits(geo.V)) return false;
+    PipePlan p;
+    return pipe_plan(geo.N, p) && kPipeQueues * 2 * p.n < (int64_t(1) << 31);
+}
+
+hipError_t launch_gsweep_pipe(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s)
+{
+    if (!a.pipe || !gsweep_pipe_applies(a.geo, gauge, a.links64)) return hipErrorNotSupported;
+    // FP64 arithmetic: the OR sweeps of the mixed driver's stage B only
+    if (algo != GF_OR && algo != GF_MICRO && algo != GF_SA) return hipErrorNotSupported;
+    if (fp64_arith && algo != GF_OR) return hipErrorNotSupported;
+    GaugeSweepArgs b = a;
+    if (!a.pipe_count || !pipe_plan(a.geo.N, b.plan, a.pipe_slack)) return hipErrorNotSupported;
+    // launch numbers 1, 2, ... below the flags' bit 31 (the flags start at 0)
+    if (*a.pipe_count >= kPipeMovedBit - 1) return hipErrorNotSupported;
+    b.pipe_launch = ++*a.pipe_count;
+    b.pipe_tdiv = make_fastdiv((uint32_t)b.plan.T);
+    b.wf = 0;
+    b.remap = 0;
+    if (b.nsb <= 0) b.nsb = 32;
+    const hipError_t e = gauge == GF_LANDAU ? launch_pipe_lf<GF_LANDAU>(algo, fp64_arith, b, s)
+                                            : launch_pipe_lf<GF_U1XU1>(algo, fp64_arith, b, s);
+    if (e == hipSuccess) g_pipe_launches++;
+    return e;
 }
 
 }  // namespace su3gf

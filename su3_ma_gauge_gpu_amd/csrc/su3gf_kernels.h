@@ -2,6 +2,7 @@
 // kernel translation units and the C ABI implementation.
 #pragma once
 #include "su3gf_device.h"
+#include "su3gf_pipe.h"
 #include "../../include/su3gf.h"
 
 namespace su3gf {
@@ -110,6 +111,16 @@ struct GaugeSweepArgs {
     int64_t wf_even0, wf_even_n, wf_even_grid;
     int64_t wf_odd0, wf_odd_n, wf_odd1, wf_odd1_n;
     uint32_t counter_odd;
+    // Pipelined sweep (launch_gsweep_pipe): the lattice's synchronisation
+    // words (allocated and zeroed by the caller, pipe_sync_words) and the
+    // host's count of launches on them; the plan and its division by T, set
+    // by the launcher
+    PipeSync *pipe;
+    uint32_t *pipe_count;  // host memory: the launches made on `pipe` so far
+    uint32_t pipe_launch;  // set by the launcher: this launch's number
+    int pipe_slack;        // extra even items before the first odd one (pipe_plan's slack)
+    PipePlan plan;
+    FastDiv pipe_tdiv;
 };
 // algo: GF_OR, GF_MICRO, GF_SA or GF_RANDOM; theta (OR only): 0 none,
 // 1 pre-update (even kernel), 2 post-update (odd kernel), 3 the read-only
@@ -129,6 +140,22 @@ hipError_t launch_gsweep(int gauge, int algo, int theta, bool fp64_arith, const 
 // 64-site blocks); the caller then launches the two kernels.
 hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, int slab,
                                    hipStream_t s);
+// The wavefront slab that stands for the pipelined sweep in the tuner's and
+// the drivers' choice (launch_gsweep_wavefront passes it on to
+// launch_gsweep_pipe).
+constexpr int kPipelinedSlab = GF_WAVEFRONT_PIPELINED;
+// One whole sweep (even then odd sites, no fused check) in one launch: the
+// pipelined schedule of su3gf_pipe.h.  Bit-identical to the two-kernel sweep
+// (odd sites draw their random numbers with counter a.counter + 1).  Landau
+// and U1xU1 on FP32 or 24-bit links, OR, microcanonical and heatbath; needs
+// a.pipe.  Returns hipErrorNotSupported where it does not apply (the caller
+// then launches the two kernels).
+hipError_t launch_gsweep_pipe(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+// whether the pipelined sweep applies to a lattice and gauge (the launcher's
+// shape conditions), so that callers allocate its words only then
+bool gsweep_pipe_applies(const Geom &geo, int gauge, bool links64);
+// pipelined sweeps launched by this process
+long long pipe_launches();
 // words of the 24-bit link copy: 9 per link, in whole tiles of 64 sites
 __host__ __device__ inline int64_t links24_words(int64_t V) { return 4 * 9 * 64 * ((V + 63) / 64); }
 // links of dtype ti and layout si (FP64 18 or 12, FP32 12) -> the 24-bit

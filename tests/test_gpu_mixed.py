@@ -623,3 +623,71 @@ def test_block_order_is_bit_identical(gf, gpu, gauge, dims):
     finally:
         _set_block_order(gf, prev)
         _set_wavefront(gf, prev_wf)
+
+
+def _pipelined_launches(gf):
+    import ctypes as C
+    n = C.c_longlong()
+    gf._lib.call("gf_pipelined_launches", C.byref(n))
+    return n.value
+
+
+def _set_pipelined_lag(gf, lag):
+    import ctypes as C
+    prev = C.c_int()
+    gf._lib.call("gf_set_pipelined_lag", int(lag), C.byref(prev))
+    return prev.value
+
+
+@pytest.mark.parametrize("gauge,dims", [(LANDAU, (8, 8, 8, 8)), (LANDAU, (16, 8, 12, 16)), (LANDAU, (24, 8, 8, 8)),
+                                        (U1XU1, (16, 16, 16, 16)), (LANDAU, (32, 32, 32, 32))])
+def test_pipelined_sweeps_are_bit_identical(gf, gpu, gauge, dims):
+    """A sweep run as the pipelined sweep (one launch: each XCD's queue of
+    even and odd 64-site blocks, an odd block starting once the even blocks
+    around it have set their flags) gives the two-kernel sweep's result bit
+    for bit: the mixed driver's FP32 phase on 24-bit and FP32 links in FP32
+    and FP64 arithmetic (the sweeps between checks every 5 sweeps), and the
+    gauge-annealing schedule on 24-bit and FP32 links (heatbath random
+    numbers per site and parity, microcanonical steps), with no extra lag and
+    with some.  The shapes give each queue one to four timeslices, include a
+    neighbour reach that is not a whole number of blocks (16x8x12x16: 96
+    sites) and the benchmark's lattice; the library's launch count shows that
+    the pipelined sweep ran."""
+    U = _cfg(gf, dims)
+    lat = gf.Lattice(dims)
+    # every setting fixed, so that no timing of candidates launches sweeps of its own
+    prev = _set_wavefront(gf, 0)
+    prev_lag = _set_pipelined_lag(gf, 0)
+    prev_order = _set_block_order(gf, 0)
+    try:
+        want = None
+        for slab, lag in ((0, 0), (gf.WAVEFRONT_PIPELINED, 0), (gf.WAVEFRONT_PIPELINED, 37),
+                          (gf.WAVEFRONT_PIPELINED, 384)):
+            _set_wavefront(gf, slab)
+            _set_pipelined_lag(gf, lag)
+            n0 = _pipelined_launches(gf)
+            got = {}
+            for mode in (gf.MIXED_GAUGE24, gf.MIXED_GAUGE32, gf.MIXED_GAUGE64):
+                d = gf.DeviceBuffer.from_numpy(U)
+                r = gf.run_mixed(lat, d, gauge, 1.7, switch_precision=1e-30, precision=1e-30, maxiter=23, mode=mode,
+                                 check32=5)
+                got[mode] = ((r.sweeps, r.checks, r.theta, r.gff), d.download())
+            for sa_mode in (1, 2):
+                d = gf.DeviceBuffer.from_numpy(U)
+                ctr = gf.run_sa(lat, d, gauge, 1.2, 0.1, 6, 2, 100, 3, 40, mode=sa_mode)
+                got["sa", sa_mode] = ((ctr,), d.download())
+            launched = _pipelined_launches(gf) - n0
+            print(dims, slab, lag, "pipelined launches", launched, flush=True)
+            if slab == 0:
+                assert launched == 0
+                want = got
+                continue
+            # 3 mixed runs x 23 sweeps (checks every 5) + 2 schedules x 6 x (1 + 2) sweeps
+            assert launched >= 3 * 18 + 2 * 18
+            for k, v in got.items():
+                assert v[0] == want[k][0], (slab, lag, k)
+                assert np.array_equal(v[1], want[k][1]), (slab, lag, k)
+    finally:
+        _set_wavefront(gf, prev)
+        _set_pipelined_lag(gf, prev_lag)
+        _set_block_order(gf, prev_order)

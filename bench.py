@@ -101,6 +101,20 @@ class Events:
         return float(ms.value)
 
 
+def sweep_path(slab: int) -> str:
+    """How the library runs a gauge-transformation sweep without a check, from
+    its wavefront choice."""
+    if slab == gf.WAVEFRONT_PIPELINED:
+        return "pipelined"
+    return f"wavefront, slabs of {slab} timeslices" if slab > 0 else "two kernels"
+
+
+def pipelined_launches() -> int:
+    n = C.c_longlong()
+    _lib.call("gf_pipelined_launches", C.byref(n))
+    return n.value
+
+
 def kernel_time_ms(lat: gf.Lattice, U: gf.DeviceBuffer, stream, nsweeps: int, variant: int,
                    gauge: int = gf.LANDAU) -> float:
     """Average duration of one parity sweep kernel as the drivers run it, from
@@ -485,9 +499,11 @@ def landau48_line(args) -> dict:
     reset()
     gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, 2, mode=args.mode, stream=stream)
     reset()
+    n_pipe = pipelined_launches()
     m = gf.run_mixed(lat64, work, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter, mode=args.mode,
                      refresh=args.refresh, check32=args.check32, reproject=args.reproject, stream=stream)
-    print(f"bench: landau 48^4 mixed {m}", file=sys.stderr, flush=True)
+    n_pipe = pipelined_launches() - n_pipe
+    print(f"bench: landau 48^4 mixed {m} ({n_pipe} pipelined sweeps)", file=sys.stderr, flush=True)
     slab, tpass = C.c_int(), C.c_int()
     _lib.call("gf_wavefront_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(slab))
     _lib.call("gf_block_order_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(tpass))
@@ -526,7 +542,8 @@ def landau48_line(args) -> dict:
                    "precision": args.precision, "switch_precision": args.switch, "mode": MODE_NAMES[args.mode]},
         "time_to_theta_mixed_s": m.ms_total / 1e3, "sweeps_mixed": m.sweeps, "sweeps_fp32": m.sweeps_fp32,
         "converged_mixed": m.converged, "gff_mixed": m.gff, "theta_mixed": m.theta,
-        "wavefront_slab": slab.value, "block_order_tpass": tpass.value,
+        "wavefront_slab": slab.value, "block_order_tpass": tpass.value, "sweep_path": sweep_path(slab.value),
+        "pipelined_sweeps_in_mixed_run": n_pipe,
         "time_to_theta_mixed_two_kernel_sweeps_s": m2.ms_total / 1e3,
         "mixed_identical_with_and_without_wavefront": (m.sweeps, m.gff, m.theta) == (m2.sweeps, m2.gff, m2.theta),
         "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps, "converged_fp64": f.converged,
@@ -658,7 +675,7 @@ def main():
         slab, tpass = C.c_int(), C.c_int()
         _lib.call("gf_wavefront_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(slab))
         _lib.call("gf_block_order_choice", C.byref(lat64.struct), gf.LANDAU, args.mode, C.byref(tpass))
-        tuning = {"wavefront_slab": slab.value, "block_order_tpass": tpass.value}
+        tuning = {"wavefront_slab": slab.value, "block_order_tpass": tpass.value, "sweep_path": sweep_path(slab.value)}
 
     # kernel timing for the roofline (on the launch stream)
     U32 = gf.DeviceBuffer(lat32.nbytes)
@@ -794,6 +811,18 @@ def main():
     if not args.no_extras:
         del src, work, U32
         line["workloads"] = extra_workloads(args, copy_streams)
+        # the same kernel's fraction where the lattice streams from HBM (48^4,
+        # 0.33 GB of 24-bit links and g beyond the 256 MB Infinity Cache): a
+        # whole sweep without a check as the mixed run makes it, per half-sweep
+        w48 = line["workloads"].get("landau_48x48x48x48", {})
+        k48 = w48.get("kernel_ms", {}).get("gauge_sweep_q24_wavefront")
+        if k48:
+            a48 = (48 ** 4 // 2) * 432 / (k48 / 1e3) / 1e9
+            line["roofline_hbm_resident"] = {
+                "bound": "hbm", "kernel": "gauge_sweep_q24, whole sweeps (" + w48.get("sweep_path", "?") + ")",
+                "dims": [48, 48, 48, 48], "achieved": a48, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": a48 / HBM_PEAK_GBS, "algorithmic_bytes_per_half_sweep": (48 ** 4 // 2) * 432,
+                "ms_per_half_sweep": k48}
     print(json.dumps(line), flush=True)
 
 

@@ -21,14 +21,15 @@ for kernel traces and hardware-counter runs:
   per-site pass) on the same buffer.
 
 Usage: python su3_ma_gauge_gpu_amd/profile_sweeps.py [--dims 32 32 32 32] [--sweeps 10] [--gauge landau|mag]
-       [--block-order 0] [--slab 0]
+       [--block-order 0] [--slab 0] [--lag 256]
 (--dims 48 48 48 48: a working set far beyond the 256 MB Infinity Cache;
 --gauge mag: the same kernels with MAG's update and check).  The
 gauge-transformation sweeps run in the XCD block order --block-order (0: one
 pass over each XCD's timeslices, n > 0: passes of n, -1: the library's tuner
 picks, timing candidates first) and the wavefront sweeps with slabs of
---slab timeslices (0: two kernels, -1: tuned); fixed settings keep the
-profiler's view to the kernels themselves.
+--slab timeslices (0: two kernels, -1: tuned, 1000000: the pipelined sweep,
+one launch per sweep, with --lag extra even blocks ahead of the odd ones);
+fixed settings keep the profiler's view to the kernels themselves.
 """
 from __future__ import annotations
 
@@ -50,9 +51,11 @@ def main() -> None:
     ap.add_argument("--gauge", choices=["landau", "mag"], default="landau")
     ap.add_argument("--block-order", type=int, default=0)
     ap.add_argument("--slab", type=int, default=0)
+    ap.add_argument("--lag", type=int, default=256)
     args = ap.parse_args()
     _lib.call("gf_set_block_order", args.block_order, None)
     _lib.call("gf_set_wavefront", args.slab, None)
+    _lib.call("gf_set_pipelined_lag", args.lag, None)
     dims = tuple(args.dims)
     gauge = gf.MAG if args.gauge == "mag" else gf.LANDAU
     kernels = gf.MAGKernelsSU3 if args.gauge == "mag" else gf.LandauKernelsSU3

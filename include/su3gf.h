@@ -312,9 +312,10 @@ gf_status gf_time_sweeps(const gf_lattice *lat, void *U, int gauge, int nsweeps,
  * GF_WAVEFRONT_PIPELINED stands for the pipelined sweep, one launch per
  * sweep: each XCD takes Nt/8 timeslices, and the odd update of a block of 64
  * sites follows the even updates of its neighbours within the same launch
- * (per-block completion flags), so that its links and neighbours come from
- * the XCD's L2.  Landau and U1xU1, Nt a multiple of 8, a timeslice's sites of
- * a parity a whole number of 64-site blocks; bit-identical results as well.
+ * (per-block completion flags), so that the second reads of its links and
+ * neighbours come from a cache instead of HBM.  Landau, Coulomb (every
+ * timeslice at once) and U1xU1; Nt a multiple of 8, a timeslice's sites of a
+ * parity a whole number of 64-site blocks; bit-identical results as well.
  * gf_set_wavefront: -1 choose by timing (default), 0 never, n > 0 slabs of
  * n timeslices (or GF_WAVEFRONT_PIPELINED); *previous (optional) receives the
  * old setting.

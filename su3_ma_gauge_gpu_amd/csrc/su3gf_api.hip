@@ -661,7 +661,7 @@ int g_slab_setting = -1;   // -1 tuned, 0 never, n > 0 fixed (gf_set_wavefront)
 int g_tpass_setting = -1;  // -1 tuned, 0 one pass, n > 0 passes of n timeslices (gf_set_block_order)
 int g_lag_setting = -1;    // -1 tuned, n >= 0 the pipelined sweep's extra lag (gf_set_pipelined_lag)
 // extra lags the tuner tries for the pipelined sweep
-constexpr int kPipeLags[] = {192, 256, 320, 384};
+constexpr int kPipeLags[] = {192, 256, 384, 512};
 
 // links: 0 FP32, 1 24-bit, 2 FP64 (with an FP64 g)
 std::array<int, 8> slab_key(const Geom &geo, int gauge, int links, bool fp64)

@@ -785,16 +785,19 @@ template <typename A, int LF> __device__ __forceinline__ void decode_link(const 
 template <typename A, int G, int ALGO, int LF>
 __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
 {
-    static_assert(G == GF_LANDAU || G == GF_U1XU1, "four directions in S");
+    static_assert(G != GF_MAG, "S a sum of links");
     static_assert(LF != 2, "FP32 or 24-bit links");
-    constexpr int kW = 4;
+    // one wave per direction of S: Coulomb's leaves out the time direction,
+    // so its odd sites' neighbours all lie in their own timeslice
+    constexpr int kW = gsweep_waves<G>();
+    constexpr int kSlices = G == GF_COULOMB ? 1 : 3;  // timeslices an odd block waits on
     __shared__ A lds[18 * kW * 64];
     if (a.state && a.state->done) return;  // converged: uniform early exit
     PipeSync *const ps = a.pipe;
     const PipePlan &pl = a.plan;
     const int queue = blockIdx.x & (kPipeQueues - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int mu = wave;
+    const int mu = wave + (4 - kW);
     const int lane = threadIdx.x & 63;
     const uint32_t launch = a.pipe_launch;  // even flags of this launch hold it
     int odd;
@@ -805,7 +808,6 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
     const int t = t0 + (int)((uint32_t)idx - qb * (uint32_t)pl.T);
     const int64_t blk = (int64_t)t * pl.Q + qb;
     const bool moved = xcc_id() != (uint32_t)queue;
-    const bool edge = t == t0 || t == t0 + pl.T - 1;
 
     // what the item reads that the launch does not write: its links and its own g
     bool active;
@@ -828,9 +830,9 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
     if (odd) {
         const int span = 2 * pl.R + 1;
         int seen_moved = 0;
-        for (int j = threadIdx.x; j < 3 * span; j += kGBlock) {
+        for (int j = threadIdx.x; j < kSlices * span; j += 64 * kW) {
             const int dtj = j / span;
-            int tt = t + dtj - 1;
+            int tt = t + dtj - (kSlices / 2);
             if (tt < 0) tt += a.geo.N[0];
             else if (tt >= a.geo.N[0]) tt -= a.geo.N[0];
             int64_t qq = (int64_t)qb + (j - dtj * span) - pl.R;
@@ -863,15 +865,17 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
     }
     {
         // the direction-mu terms as direction_terms forms them; the neighbours
-        // of an odd site are even sites written in this launch
-        const bool coh_g = odd && (coh || (mu == 0 && edge));
+        // of an odd site are even sites written in this launch, by this XCD
+        // except across the queue's first and last timeslices
+        const bool coh_y = odd && (coh || (mu == 0 && t == t0 + pl.T - 1));
+        const bool coh_z = odd && (coh || (mu == 0 && t == t0));
         M3<A> X, Z, T, W;
         decode_link<A, LF>(lx, W);
-        if (coh_g) load_g<A, LF, kSc1>(fg, y_off, T);
+        if (coh_y) load_g<A, LF, kSc1>(fg, y_off, T);
         else load_g<A, LF>(fg, y_off, T);
         third_row(T);
         prod_adag(X, W, T);  // U0_mu(x) g(y)^dagger
-        if (coh_g) load_g<A, LF, kSc1>(fg, z_off, T);
+        if (coh_z) load_g<A, LF, kSc1>(fg, z_off, T);
         else load_g<A, LF>(fg, z_off, T);
         decode_link<A, LF>(lz, W);
         third_row(W);
@@ -895,9 +899,11 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
                 const int k = 2 * (3 * i + j);
                 A v[2];
 #pragma unroll
-                for (int r = 0; r < 2; r++) {
+                for (int r = 0; r < 2; r++) {  // in the order of mu, as k_gsweep adds them
                     const A *b = lds + (k + r) * kW * 64 + lane;
-                    v[r] = ((b[0] + b[64]) + b[128]) + b[192];
+                    v[r] = b[0] + b[64];
+#pragma unroll
+                    for (int w = 2; w < kW; w++) v[r] += b[64 * w];
                 }
                 S.e[i][j] = cx(v[0], v[1]);
             }
@@ -1310,7 +1316,7 @@ hipError_t launch_gsweep_wavefront(int gauge, int algo, bool fp64_arith, const G
 namespace {
 template <typename A, int G, int ALGO, int LF> hipError_t launch_k_pipe(const GaugeSweepArgs &a, hipStream_t s)
 {
-    const dim3 grid((unsigned)(kPipeQueues * 2 * a.plan.n)), block(kGBlock);
+    const dim3 grid((unsigned)(kPipeQueues * 2 * a.plan.n)), block(64 * gsweep_waves<G>());
     hipLaunchKernelGGL((k_gsweep_pipe<A, G, ALGO, LF>), grid, block, 0, s, a);
     return hipGetLastError();
 }
@@ -1335,7 +1341,9 @@ long long pipe_launches() { return g_pipe_launches.load(); }
 
 bool gsweep_pipe_applies(const Geom &geo, int gauge, bool links64)
 {
-    if (links64 || geo.slice_bps != 0 || (gauge != GF_LANDAU && gauge != GF_U1XU1)) return false;
+    if (links64 || gauge == GF_MAG) return false;
+    // Coulomb's per-timeslice blocks: whole 64-site blocks only (then they are the plain ones)
+    if (geo.slice_bps != 0 && geo.slice_bps * 64 != geo.slice_half) return false;
     if (!gauge_sweep_fits(geo.V)) return false;
     PipePlan p;
     return pipe_plan(geo.N, p) && kPipeQueues * 2 * p.n < (int64_t(1) << 31);
@@ -1356,8 +1364,9 @@ hipError_t launch_gsweep_pipe(int gauge, int algo, bool fp64_arith, const GaugeS
     b.wf = 0;
     b.remap = 0;
     if (b.nsb <= 0) b.nsb = 32;
-    const hipError_t e = gauge == GF_LANDAU ? launch_pipe_lf<GF_LANDAU>(algo, fp64_arith, b, s)
-                                            : launch_pipe_lf<GF_U1XU1>(algo, fp64_arith, b, s);
+    const hipError_t e = gauge == GF_LANDAU    ? launch_pipe_lf<GF_LANDAU>(algo, fp64_arith, b, s)
+                         : gauge == GF_COULOMB ? launch_pipe_lf<GF_COULOMB>(algo, fp64_arith, b, s)
+                                               : launch_pipe_lf<GF_U1XU1>(algo, fp64_arith, b, s);
     if (e == hipSuccess) g_pipe_launches++;
     return e;
 }

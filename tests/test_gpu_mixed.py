@@ -640,7 +640,9 @@ def _set_pipelined_lag(gf, lag):
 
 
 @pytest.mark.parametrize("gauge,dims", [(LANDAU, (8, 8, 8, 8)), (LANDAU, (16, 8, 12, 16)), (LANDAU, (24, 8, 8, 8)),
-                                        (U1XU1, (16, 16, 16, 16)), (LANDAU, (32, 32, 32, 32))])
+                                        (U1XU1, (16, 16, 16, 16)), (LANDAU, (32, 32, 32, 32)),
+                                        (COULOMB, (8, 8, 8, 16)), (COULOMB, (16, 8, 12, 16)),
+                                        (COULOMB, (64, 32, 32, 32))])
 def test_pipelined_sweeps_are_bit_identical(gf, gpu, gauge, dims):
     """A sweep run as the pipelined sweep (one launch: each XCD's queue of
     even and odd 64-site blocks, an odd block starting once the even blocks
@@ -649,10 +651,11 @@ def test_pipelined_sweeps_are_bit_identical(gf, gpu, gauge, dims):
     and FP64 arithmetic (the sweeps between checks every 5 sweeps), and the
     gauge-annealing schedule on 24-bit and FP32 links (heatbath random
     numbers per site and parity, microcanonical steps), with no extra lag and
-    with some.  The shapes give each queue one to four timeslices, include a
+    with some.  The shapes give each queue one to eight timeslices, include a
     neighbour reach that is not a whole number of blocks (16x8x12x16: 96
-    sites) and the benchmark's lattice; the library's launch count shows that
-    the pipelined sweep ran."""
+    sites), the benchmark's lattice and Coulomb's 32^3 x 64 (every timeslice
+    at once, its odd blocks waiting on their own timeslice only); the
+    library's launch count shows that the pipelined sweep ran."""
     U = _cfg(gf, dims)
     lat = gf.Lattice(dims)
     # every setting fixed, so that no timing of candidates launches sweeps of its own

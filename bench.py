@@ -232,9 +232,18 @@ def coulomb_line(args) -> dict:
     gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.switch, args.precision, 2, mode=args.mode, stream=stream)
     for _ in range(args.warmup):
         mixed()
+    n_pipe = pipelined_launches()
     rs = [mixed() for _ in range(args.steps)]
+    n_pipe = (pipelined_launches() - n_pipe) // max(1, args.steps)
     r = rs[-1]
     th_m, g_m = slice_quality(lat, work, stream)
+    slab = C.c_int()
+    _lib.call("gf_wavefront_choice", C.byref(lat.struct), gf.COULOMB, args.mode, C.byref(slab))
+    # the same run with every sweep as two kernels
+    prev = C.c_int()
+    _lib.call("gf_set_wavefront", 0, C.byref(prev))
+    r2 = mixed()
+    _lib.call("gf_set_wavefront", prev.value, None)
     # the workload's stated schedule: FP32 12-real links until theta < 1e-6,
     # then FP64 sweeps (on the gauge transformation)
     reset()
@@ -252,6 +261,9 @@ def coulomb_line(args) -> dict:
                        "switch_precision": args.switch, "mode": MODE_NAMES[args.mode]})
     line.update({
         "time_to_theta_mixed_s": ms / 1e3, "sweeps_mixed": r.sweeps, "sweeps_fp32": r.sweeps_fp32,
+        "sweep_path": sweep_path(slab.value), "pipelined_sweeps_in_mixed_run": n_pipe,
+        "time_to_theta_mixed_two_kernel_sweeps_s": r2.ms_total / 1e3,
+        "mixed_identical_with_and_without_wavefront": (r.sweeps, r.gff, r.theta) == (r2.sweeps, r2.gff, r2.theta),
         "time_to_theta_fp64_s": f.ms_total / 1e3, "sweeps_fp64": f.sweeps,
         "speedup_mixed_vs_fp64": f.ms_total / ms,
         "max_slice_theta_mixed": float(th_m.max()), "max_slice_theta_fp64": float(th_f.max()),

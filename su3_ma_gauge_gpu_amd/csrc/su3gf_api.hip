@@ -702,7 +702,7 @@ float time_gsweep(const GaugeSweepArgs &t, int gauge, bool fp64, int slab, int n
     return ms / n;
 }
 
-// times a few OR sweeps of each candidate on a scratch copy of g: first the
+// times OR sweeps of each candidate on a scratch copy of g: first the
 // block order of the two-kernel sweep (passes of each divisor of the XCD's
 // timeslices), then, with that order, the wavefront slabs; a candidate must
 // beat the incumbent by 3% (launch count, noise).  Never fails (the defaults
@@ -722,14 +722,20 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         t.partial = nullptr;
         t.param = 1.7f;
         t.tpass = g_tpass_setting >= 0 ? g_tpass_setting : 0;
+        // a first measurement brings clocks and caches to a steady state and
+        // is dropped; then every candidate is timed over about 3 ms of sweeps
+        // (at least 3), which separates candidates a few percent apart
         float two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+        two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+        const int n = two > 0 ? std::min(40, std::max(3, (int)(3.f / two))) : 3;
+        if (n > 3) two = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
         const int spx = xcd_slices(geo);
         if (g_tpass_setting < 0 && spx > 1 && two > 0) {
             const float one_pass = two;
             for (int P = 1; P < spx; P++) {
                 if (spx % P != 0) continue;
                 t.tpass = P;
-                const float ms = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+                const float ms = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
                 if (ms > 0 && ms < two && ms < 0.97f * one_pass) {
                     two = ms;
                     tu.tpass = P;
@@ -740,7 +746,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         float best_ms = two;
         for (int c : {2, 3, 4, 6, 8, 12}) {
             if (c >= geo.N[0] || two <= 0) break;
-            const float ms = time_gsweep(t, gauge, fp64, c, 3, s, ev.a, ev.b);
+            const float ms = time_gsweep(t, gauge, fp64, c, n, s, ev.a, ev.b);
             if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
                 best_ms = ms;
                 tu.slab = c;
@@ -754,7 +760,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
             for (int lag : kPipeLags) {
                 if (g_lag_setting >= 0 && lag != kPipeLags[0]) break;
                 t.pipe_slack = g_lag_setting >= 0 ? g_lag_setting : lag;
-                const float ms = time_gsweep(t, gauge, fp64, kPipelinedSlab, 3, s, ev.a, ev.b);
+                const float ms = time_gsweep(t, gauge, fp64, kPipelinedSlab, n, s, ev.a, ev.b);
                 if (ms > 0 && ms < best_ms && ms < 0.97f * two && pb.check(s) == GF_OK) {
                     best_ms = ms;
                     tu.slab = kPipelinedSlab;

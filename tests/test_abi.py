@@ -214,3 +214,25 @@ def test_fastdiv_formula(d):
     hi = (ns * np.uint64(m)) >> np.uint64(32)
     q = (hi + ns) >> np.uint64(s)
     assert np.array_equal(q, ns // np.uint64(d))
+
+
+def test_sweep_path_settings(gf):
+    """The tuning settings of the gauge-transformation sweep are host-side:
+    gf_set_wavefront, gf_set_block_order and gf_set_pipelined_lag take -1
+    (tuned) or a fixed value, return the previous one and reject the rest;
+    gf_pipelined_launches counts the pipelined sweeps (none without a GPU)."""
+    lib = gf._lib.load()
+    prev = C.c_int()
+    for name, good in (("gf_set_wavefront", [gf.WAVEFRONT_PIPELINED, 3, 0]), ("gf_set_block_order", [2, 0]),
+                       ("gf_set_pipelined_lag", [256, 0])):
+        fn = getattr(lib, name)
+        assert fn(-1, C.byref(prev)) == 0
+        first = prev.value
+        for v in good:
+            assert fn(v, None) == 0
+            assert fn(v, C.byref(prev)) == 0 and prev.value == v
+        assert fn(-2, None) == 2
+        assert fn(first, None) == 0
+    n = C.c_longlong(-1)
+    assert lib.gf_pipelined_launches(C.byref(n)) == 0 and n.value == 0
+    assert lib.gf_pipelined_launches(None) == 2

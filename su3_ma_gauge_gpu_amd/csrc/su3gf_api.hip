@@ -721,22 +721,24 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         t.state = nullptr;
         t.partial = nullptr;
         t.param = 1.7f;
-        t.tpass = g_tpass_setting >= 0 ? g_tpass_setting : 0;
-        // a first measurement brings clocks and caches to a steady state and
-        // is dropped; then every candidate is timed over about 3 ms of sweeps
-        // (at least 3), which separates candidates a few percent apart
-        float two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
-        two = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
-        const int n = two > 0 ? std::min(40, std::max(3, (int)(3.f / two))) : 3;
-        if (n > 3) two = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
+        const int base_tpass = g_tpass_setting >= 0 ? g_tpass_setting : 0;
+        t.tpass = base_tpass;
+        // about 20 ms of sweeps first, dropped: right after an allocation the
+        // clocks and the Infinity Cache are not yet at a sustained run's
+        // steady state, which would favour whatever is timed later; then
+        // each candidate over about 8 ms of sweeps (at least 3)
+        const float est = time_gsweep(t, gauge, fp64, 0, 3, s, ev.a, ev.b);
+        if (est > 0) (void)time_gsweep(t, gauge, fp64, 0, std::min(400, std::max(3, (int)(20.f / est))), s, ev.a, ev.b);
+        const int n = est > 0 ? std::min(100, std::max(3, (int)(8.f / est))) : 3;
+        const float base = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
+        float two = base;
         const int spx = xcd_slices(geo);
-        if (g_tpass_setting < 0 && spx > 1 && two > 0) {
-            const float one_pass = two;
+        if (g_tpass_setting < 0 && spx > 1 && base > 0) {
             for (int P = 1; P < spx; P++) {
                 if (spx % P != 0) continue;
                 t.tpass = P;
                 const float ms = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
-                if (ms > 0 && ms < two && ms < 0.97f * one_pass) {
+                if (ms > 0 && ms < two && ms < 0.97f * base) {
                     two = ms;
                     tu.tpass = P;
                 }
@@ -745,7 +747,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         }
         float best_ms = two;
         for (int c : {2, 3, 4, 6, 8, 12}) {
-            if (c >= geo.N[0] || two <= 0) break;
+            if (c >= geo.N[0] || base <= 0) break;
             const float ms = time_gsweep(t, gauge, fp64, c, n, s, ev.a, ev.b);
             if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
                 best_ms = ms;
@@ -754,7 +756,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         }
         // the pipelined sweep, on synchronisation words of its own, for each extra lag
         PipeBuf pb;
-        if (two > 0 && pb.init(geo, gauge, ga.links64 != 0, s) == hipSuccess && pb.get()) {
+        if (base > 0 && pb.init(geo, gauge, ga.links64 != 0, s) == hipSuccess && pb.get()) {
             t.pipe = pb.get();
             t.pipe_count = &pb.count;
             for (int lag : kPipeLags) {
@@ -767,8 +769,27 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
                     tu.pipe_slack = t.pipe_slack;
                 }
             }
-            t.pipe = nullptr;
         }
+        // confirm a choice other than the default (two kernels, the base
+        // block order) by timing the two alternately, twice each: it must
+        // still win by 3%, or the default stays
+        if ((tu.slab != 0 || tu.tpass != base_tpass) && base > 0) {
+            GaugeSweepArgs d = t, w = t;
+            d.tpass = base_tpass;
+            w.tpass = tu.tpass;
+            w.pipe_slack = tu.pipe_slack;
+            float d_ms = 0, w_ms = 0;
+            for (int r = 0; r < 2; r++) {
+                const float a1 = time_gsweep(d, gauge, fp64, 0, n, s, ev.a, ev.b);
+                const float b1 = time_gsweep(w, gauge, fp64, tu.slab, n, s, ev.a, ev.b);
+                if (a1 <= 0 || b1 <= 0) d_ms = -1e30f;
+                d_ms += a1;
+                w_ms += b1;
+            }
+            if (!(d_ms > 0 && w_ms < 0.97f * d_ms && (tu.slab != kPipelinedSlab || pb.check(s) == GF_OK)))
+                tu = GsweepTuning{};
+        }
+        t.pipe = nullptr;
     }
     (void)hipStreamSynchronize(s);
     (void)hipGetLastError();
@@ -1660,6 +1681,12 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
         // sweeps without a check: as a wavefront over timeslices where that
         // pays (lattices beyond the Infinity Cache)
         const int slab = wavefront_slab(ga, gauge, mode == GF_MIXED_GAUGE64, s);
+        if (mode != GF_MIXED_GAUGE64) {
+            // stage B's sweeps (below), tuned with the FP32 phase's so that a
+            // later run of this kind of lattice times no candidates
+            GaugeSweepArgs gb = ga;
+            (void)wavefront_slab(gb, gauge, true, s);
+        }
         auto plain = [&](int) {
             if (slab <= 0) return hipErrorNotSupported;
             return launch_gsweep_wavefront(gauge, GF_OR, mode == GF_MIXED_GAUGE64, ga, slab, s);

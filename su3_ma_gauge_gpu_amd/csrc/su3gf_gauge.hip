@@ -726,6 +726,10 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
 // another XCD -- the time-direction neighbour in the next queue's or the
 // previous queue's timeslice, and any block whose flag says it ran elsewhere
 // (never seen on this hardware, where block b runs on XCD b % 8, but handled).
+// A CU's L1 holds no stale even g: the only lines of it the CU can have read
+// before the flags are those of its own even items, whose stores updated it.
+// No device-scope fences: a device-scope acquire (buffer_inv sc1) after the
+// wait made the sweep 3x slower (it drops the L2 lines the odd items reuse).
 
 // the XCD this wave runs on (hardware register XCC_ID, bits 3:0)
 __device__ __forceinline__ uint32_t xcc_id()

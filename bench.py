@@ -181,6 +181,27 @@ def _stream():
     return stream
 
 
+def pretune(lat: gf.Lattice, src: gf.DeviceBuffer, gauge: int, modes, stream, sa_mode=None) -> float:
+    """The library's one-time tuning for a kind of lattice, outside the timed
+    runs: the first mixed run of a (shape, gauge, link form) times candidate
+    sweep paths for its FP32 phase, its FP64-arithmetic stage and its FP64
+    phase (about 0.1-0.3 s at 32^4), and the first annealing schedule those of
+    its sweeps.  A two-sweep run_mixed per mode in `modes` (and a two-step
+    schedule in `sa_mode`) on a scratch copy of `src` does all of that.
+    Returns the seconds it took, which the lines report as `tuning_s`."""
+    scratch = gf.DeviceBuffer(lat.nbytes)
+    _lib.call("gf_synchronize")
+    t0 = time.perf_counter()
+    for m in modes:
+        _lib.call("gf_memcpy_async", C.c_void_p(scratch.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
+        gf.run_mixed(lat, scratch, gauge, 1.7, 1e-12, 1e-12, 2, mode=m, stream=stream)
+    if sa_mode is not None:
+        _lib.call("gf_memcpy_async", C.c_void_p(scratch.ptr), C.c_void_p(src.ptr), C.c_size_t(lat.nbytes), stream)
+        gf.run_sa(lat, scratch, gauge, 1.45, 0.01, 2, 3, 100, mode=sa_mode, stream=stream)
+    _lib.call("gf_synchronize")
+    return time.perf_counter() - t0
+
+
 def _base_line(metric, value, unit, ms_per_step, steps, warmup, config, higher=True):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
@@ -245,7 +266,9 @@ def coulomb_line(args) -> dict:
     r2 = mixed()
     _lib.call("gf_set_wavefront", prev.value, None)
     # the workload's stated schedule: FP32 12-real links until theta < 1e-6,
-    # then FP64 sweeps (on the gauge transformation)
+    # then FP64 sweeps (on the gauge transformation); its mode's first run
+    # would tune its sweep paths inside its timing
+    tuning_s = pretune(lat, src, gf.COULOMB, [gf.MIXED_GAUGE32], stream) if args.mode != gf.MIXED_GAUGE32 else 0.0
     reset()
     st = gf.run_mixed(lat, work, gf.COULOMB, args.omega, args.early_switch, args.precision, args.maxiter,
                       mode=gf.MIXED_GAUGE32, check32=args.check32, stream=stream)
@@ -272,7 +295,7 @@ def coulomb_line(args) -> dict:
         "mixed_fp32_links_switch_1e-6": {
             "time_to_theta_s": st.ms_total / 1e3, "sweeps": st.sweeps, "sweeps_fp32": st.sweeps_fp32,
             "speedup_vs_fp64": f.ms_total / st.ms_total, "max_slice_theta": float(th_s.max()),
-            "max_slice_gff_rel_diff": float(np.max(np.abs(g_s - g_f) / np.abs(g_f)))},
+            "max_slice_gff_rel_diff": float(np.max(np.abs(g_s - g_f) / np.abs(g_f))), "tuning_s": tuning_s},
     })
     return line
 
@@ -342,6 +365,9 @@ def mag_line(args, L: int, maxiter: int, runs: str = "both") -> dict:
 
     mixed = fp64 = m = f = None
     fixed = []  # (name, buffer holding that run's MAG result)
+    # the first mixed run of this lattice would tune its sweep paths (the FP32
+    # phase's, its FP64-arithmetic stage's and the FP64 phase's) inside its timing
+    tuning_s = pretune(lat, src, gf.MAG, [args.mode], stream) if runs in ("both", "mixed") else 0.0
     if runs in ("both", "mixed"):
         # the FP32 phase to 1e-6 (gf_run_mixed with precision 1e-6 ends on its
         # first FP64 check), then FP64 sweeps to the precision as the mixed
@@ -388,7 +414,7 @@ def mag_line(args, L: int, maxiter: int, runs: str = "both") -> dict:
                       {"workload": f"mag-or-{L}x{L}x{L}x{L}-mixed+u1xu1", "dims": list(dims), "sigma": 0.35,
                        "seed": 1, "haar_seed": 2, "omega": args.omega, "precision": prec, "switch_precision": 1e-6,
                        "mode": MODE_NAMES[args.mode], "u1xu1_sweeps": args.u1xu1_sweeps, "runs": runs})
-    line.update({"mixed": mixed, "fp64": fp64, "u1xu1": u1})
+    line.update({"mixed": mixed, "fp64": fp64, "u1xu1": u1, "tuning_s": tuning_s})
     if mixed is not None and fp64 is not None:
         both = mixed["converged"] and fp64["converged"]
         line.update({
@@ -452,6 +478,7 @@ def gribov_line(args, streams=None, graph=False) -> dict:
         streams = [_stream() for _ in range(args.copies)]
     sizes = [16, 20, 24, 32]
     total_su, total_s, per_size = 0, 0.0, {}
+    tuning_s = 0.0
     haar = 10
     for L in sizes:
         dims = (L, L, L, L)
@@ -460,6 +487,9 @@ def gribov_line(args, streams=None, graph=False) -> dict:
         hs = list(range(haar, haar + args.copies))
         haar += args.copies
         bufs = [gf.DeviceBuffer.from_numpy(gf.make_config(dims, 0.3, 1, h)) for h in hs]
+        # the first annealing schedule and mixed run of a size tune their sweep
+        # paths; done here, on a scratch copy, outside the timed region
+        tuning_s += pretune(lats[0], bufs[0], gf.LANDAU, [args.mode], streams[0], sa_mode=args.sa_mode)
         _lib.call("gf_synchronize")
         t0 = time.perf_counter()  # the inputs were generated and uploaded before t0
         sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject,
@@ -484,7 +514,7 @@ def gribov_line(args, streams=None, graph=False) -> dict:
                        "sa_graph": graph,
                        "sigma": 0.3, "seed": 1, "haar_seeds": [10, haar - 1], "sa": [1.45, 0.01, args.sa_steps, 3],
                        "precision": args.precision, "mode": MODE_NAMES[args.mode]})
-    line.update({"total_s": total_s, "per_size": per_size})
+    line.update({"total_s": total_s, "per_size": per_size, "tuning_s": tuning_s})
     return line
 
 
@@ -671,7 +701,11 @@ def main():
     # single-precision modes
     ns = None
     modes = {}
+    tuning_s = 0.0
     if not args.no_fp64:
+        # the other modes' first runs would time their candidate sweep paths
+        # inside their own timing (the warm-up did this for args.mode)
+        tuning_s = pretune(lat64, src, gf.LANDAU, [m for m in (1, 2, 3) if m != args.mode], stream)
         ns = mixed(args.mode, args.early_switch)
         print(f"bench: mixed, switch {args.early_switch:g}: {ns}", file=sys.stderr, flush=True)
         for m in (0, 1, 2, 3):
@@ -797,6 +831,7 @@ def main():
             "time_to_theta_s": ns.ms_total / 1e3, "sweeps": ns.sweeps, "sweeps_fp32": ns.sweeps_fp32,
             "gff": ns.gff, "speedup_vs_fp64": fp64.ms_total / ns.ms_total},
         "mixed_modes": modes,
+        "mixed_modes_tuning_s": tuning_s,
         "dropin_run_or": None if dropin is None else {
             "time_to_theta_s": dropin.ms_total / 1e3, "sweeps": dropin.sweeps, "gff": dropin.gff,
             "theta": dropin.theta, "site_updates_per_s": dropin.sweeps * V / (dropin.ms_total / 1e3)},

@@ -931,18 +931,21 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
 // g(x+mu)^dagger and U_dn = g(x-mu) U0_mu(x-mu) g(x)^dagger in registers and
 // applies every SU(2) step to them (U_up <- G U_up, U_dn <- U_dn G^dagger), as
 // the link sweep does.  Per subgroup each wave exchanges its direction's
-// (D, Im c, Re c), up and down link added, through LDS; every wave adds the
-// four in the order mu = 0..3 and computes the same update from the same
-// Philox stream, so they agree bit for bit; wave 0 also updates and stores
-// g(x).
+// (D, Im c, Re c), up and down link added, through LDS; wave 0 adds the four
+// in the order mu = 0..3, computes the update (drawing the site's random
+// numbers, where the update takes any) and hands it to the other waves
+// through LDS: a square root, a division and two inverse square roots per
+// subgroup that the other three waves no longer repeat.  Wave 0 also updates
+// and stores g(x).
 // theta (THETA = 1 before, 2 after the update) comes from the links as in the
 // link sweep's MAG check.
 template <typename A, int ALGO, int THETA, int LF>
 __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
 {
     constexpr int kX = 4 * 3 * 64;  // one exchange: [direction 4][D, Im c, Re c][lane 64]
-    __shared__ A lds[2 * kX + (THETA ? kThetaVals * 4 * 64 : 0)];
-    A *const tlds = lds + 2 * kX;
+    __shared__ A lds[2 * kX + 4 * 64 + (THETA ? kThetaVals * 4 * 64 : 0)];
+    A *const qlds = lds + 2 * kX;  // the update: [q0..q3][lane 64]
+    A *const tlds = qlds + 4 * 64;
     static_assert(LF != 2 || sizeof(A) == 8, "FP64 storage takes FP64 arithmetic");
     if (a.state && a.state->done) return;  // converged: uniform early exit
     if (THETA == 3 && !a.state->near) return;
@@ -966,8 +969,8 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
     M3<A> Gx;
     load_g<A, LF>(fg, site_off, Gx);
     third_row(Gx);
-    Philox rng;  // the same stream in every wave
-    if (ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) site_rng(a, p, parity, rng);
+    Philox rng;  // wave 0 computes the updates
+    if ((ALGO == GF_SA || ALGO == GF_SR || ALGO == GF_RANDOM) && mu == 0) site_rng(a, p, parity, rng);
 
     M3<A> up, dn;
     {
@@ -1019,17 +1022,28 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_mag(GaugeSweepArgs a)
 #pragma unroll
         for (int k = 0; k < 3; k++) x[(mu * 3 + k) * 64 + lane] = pd[k];
         __syncthreads();
-        A cs[4][4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) cs[m][k] = x[(m * 3 + k) * 64 + lane];
-            cs[m][3] = 0;
-        }
-        buf ^= 1;  // the next exchange uses the other buffer: a wave that races
-                   // ahead writes there, never into the one still being read
+        buf ^= 1;  // the next exchange uses the other buffer
         A q[4];
-        site_update_dirs<A, GF_MAG, ALGO, false>(cs, q, a.param, rng);
+        if (mu == 0) {
+            A cs[4][4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) cs[m][k] = x[(m * 3 + k) * 64 + lane];
+                cs[m][3] = 0;
+            }
+            site_update_dirs<A, GF_MAG, ALGO, false>(cs, q, a.param, rng);
+#pragma unroll
+            for (int k = 0; k < 4; k++) qlds[k * 64 + lane] = q[k];
+        }
+        // the other waves read it after this barrier; wave 0 writes the next
+        // one only after the next exchange's barrier, which they reach after
+        // reading this one
+        __syncthreads();
+        if (mu != 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = qlds[k * 64 + lane];
+        }
         left_su2<A, I, J>(up, q);
         const Hd<A> h = su2_dag(q);
 #pragma unroll

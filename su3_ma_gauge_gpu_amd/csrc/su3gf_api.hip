@@ -6,6 +6,7 @@
 // a message retrievable through gf_last_error().
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <array>
@@ -707,10 +708,26 @@ float time_gsweep(const GaugeSweepArgs &t, int gauge, bool fp64, int slab, int n
 // timeslices), then, with that order, the wavefront slabs; a candidate must
 // beat the incumbent by 3% (launch count, noise).  Never fails (the defaults
 // on any error).
+// SU3GF_TUNE_LOG=1 in the environment: the tuner prints every candidate's
+// time per sweep and its choice to stderr
+bool tune_log()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("SU3GF_TUNE_LOG");
+        return e && *e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+}
+
 GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
 {
     const Geom &geo = ga.geo;
     GsweepTuning tu;
+    const bool log = tune_log();
+    if (log)
+        std::fprintf(stderr, "su3gf tune: %dx%dx%dx%d gauge %d links %s, %s arithmetic\n", geo.N[0], geo.N[1],
+                     geo.N[2], geo.N[3], gauge, ga.links64 ? "fp64" : ga.links24 ? "24-bit" : "fp32",
+                     fp64 ? "fp64" : "fp32");
     Scratch g;
     EventPair ev;
     const size_t gbytes = (ga.links64 ? sizeof(double) : sizeof(float)) * gauge_floats(geo.V);
@@ -731,6 +748,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         if (est > 0) (void)time_gsweep(t, gauge, fp64, 0, std::min(400, std::max(3, (int)(20.f / est))), s, ev.a, ev.b);
         const int n = est > 0 ? std::min(100, std::max(3, (int)(8.f / est))) : 3;
         const float base = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
+        if (log) std::fprintf(stderr, "su3gf tune:   two kernels, one pass: %.4f ms per sweep (%d sweeps)\n", base, n);
         float two = base;
         const int spx = xcd_slices(geo);
         if (g_tpass_setting < 0 && spx > 1 && base > 0) {
@@ -738,6 +756,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
                 if (spx % P != 0) continue;
                 t.tpass = P;
                 const float ms = time_gsweep(t, gauge, fp64, 0, n, s, ev.a, ev.b);
+                if (log) std::fprintf(stderr, "su3gf tune:   two kernels, passes of %d timeslices: %.4f\n", P, ms);
                 if (ms > 0 && ms < two && ms < 0.97f * base) {
                     two = ms;
                     tu.tpass = P;
@@ -749,6 +768,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
         for (int c : {2, 3, 4, 6, 8, 12}) {
             if (c >= geo.N[0] || base <= 0) break;
             const float ms = time_gsweep(t, gauge, fp64, c, n, s, ev.a, ev.b);
+            if (log) std::fprintf(stderr, "su3gf tune:   wavefront, slabs of %d: %.4f\n", c, ms);
             if (ms > 0 && ms < best_ms && ms < 0.97f * two) {
                 best_ms = ms;
                 tu.slab = c;
@@ -763,6 +783,7 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
                 if (g_lag_setting >= 0 && lag != kPipeLags[0]) break;
                 t.pipe_slack = g_lag_setting >= 0 ? g_lag_setting : lag;
                 const float ms = time_gsweep(t, gauge, fp64, kPipelinedSlab, n, s, ev.a, ev.b);
+                if (log) std::fprintf(stderr, "su3gf tune:   pipelined, lag %d: %.4f\n", t.pipe_slack, ms);
                 if (ms > 0 && ms < best_ms && ms < 0.97f * two && pb.check(s) == GF_OK) {
                     best_ms = ms;
                     tu.slab = kPipelinedSlab;
@@ -786,13 +807,23 @@ GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStre
                 d_ms += a1;
                 w_ms += b1;
             }
-            if (!(d_ms > 0 && w_ms < 0.97f * d_ms && (tu.slab != kPipelinedSlab || pb.check(s) == GF_OK)))
-                tu = GsweepTuning{};
+            const bool keep = d_ms > 0 && w_ms < 0.97f * d_ms && (tu.slab != kPipelinedSlab || pb.check(s) == GF_OK);
+            if (log)
+                std::fprintf(stderr, "su3gf tune:   confirmation, alternately: default %.4f, choice %.4f: %s\n",
+                             d_ms / 2, w_ms / 2, keep ? "kept" : "default kept");
+            if (!keep) tu = GsweepTuning{};
         }
         t.pipe = nullptr;
     }
     (void)hipStreamSynchronize(s);
     (void)hipGetLastError();
+    if (log) {
+        char path[64];
+        if (tu.slab == kPipelinedSlab) std::snprintf(path, sizeof path, "pipelined, lag %d", tu.pipe_slack);
+        else if (tu.slab > 0) std::snprintf(path, sizeof path, "wavefront, slabs of %d", tu.slab);
+        else std::snprintf(path, sizeof path, "two kernels");
+        std::fprintf(stderr, "su3gf tune:   chosen: %s, block order %d\n", path, tu.tpass);
+    }
     return tu;
 }
 

@@ -457,7 +457,46 @@ def workload_gribov(args):
     streams = [_stream() for _ in range(args.copies)]
     line = gribov_line(args, streams)
     line["with_sa_graph"] = gribov_line(args, streams, graph=True)
+    fp64 = gribov_fp64(args, streams[0])
+    line["fp64_reference_algorithm"] = fp64
+    line["speedup_vs_fp64"] = fp64["total_s"] / line["total_s"]
     print(json.dumps(line), flush=True)
+
+
+def gribov_fp64(args, stream) -> dict:
+    """The gauge copies of gribov_line as the reference apps run them, in FP64
+    (the annealing on FP64 link sweeps, run_sa mode 0, then the FP64 OR of
+    gf_run_fast), one copy after the other on one stream, each size after an
+    untimed first run of its kind; wall time from the first sweep to the last
+    copy's result, inputs resident."""
+    sizes = [16, 20, 24, 32]
+    total_s, per_size, haar = 0.0, {}, 10
+    for L in sizes:
+        dims = (L, L, L, L)
+        lat = gf.Lattice(dims)
+        hs = list(range(haar, haar + args.copies))
+        haar += args.copies
+        bufs = [gf.DeviceBuffer.from_numpy(gf.make_config(dims, 0.3, 1, h)) for h in hs]
+        scratch = gf.DeviceBuffer(lat.nbytes)
+        _lib.call("gf_memcpy_async", C.c_void_p(scratch.ptr), C.c_void_p(bufs[0].ptr), C.c_size_t(lat.nbytes), stream)
+        gf.run_sa(lat, scratch, gf.LANDAU, 1.45, 0.01, 2, 3, args.reproject, mode=0, stream=stream)
+        gf.run_fast(lat, scratch, gf.LANDAU, args.omega, args.precision, 2, args.reproject, stream=stream)
+        _lib.call("gf_synchronize")
+        t0 = time.perf_counter()
+        gffs, sweeps = [], []
+        for h, buf in zip(hs, bufs):
+            gf.run_sa(lat, buf, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject, seed=1,
+                      counter=(h - 10) * 100000, mode=0, stream=stream)
+            r = gf.run_fast(lat, buf, gf.LANDAU, args.omega, args.precision, args.maxiter, args.reproject,
+                            stream=stream)
+            gffs.append(r.gff)
+            sweeps.append(r.sweeps)
+        _lib.call("gf_synchronize")
+        size_s = time.perf_counter() - t0
+        total_s += size_s
+        per_size[f"{L}^4"] = {"best_gff": max(gffs), "wall_s": size_s, "or_sweeps": sweeps}
+        print(f"bench: fp64 copies {L}^4: {size_s:.2f} s, best gff {max(gffs)!r}", file=sys.stderr, flush=True)
+    return {"total_s": total_s, "per_size": per_size}
 
 
 def gribov_line(args, streams=None, graph=False) -> dict:

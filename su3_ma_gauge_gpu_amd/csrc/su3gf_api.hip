@@ -703,11 +703,6 @@ float time_gsweep(const GaugeSweepArgs &t, int gauge, bool fp64, int slab, int n
     return ms / n;
 }
 
-// times OR sweeps of each candidate on a scratch copy of g: first the
-// block order of the two-kernel sweep (passes of each divisor of the XCD's
-// timeslices), then, with that order, the wavefront slabs; a candidate must
-// beat the incumbent by 3% (launch count, noise).  Never fails (the defaults
-// on any error).
 // SU3GF_TUNE_LOG=1 in the environment: the tuner prints every candidate's
 // time per sweep and its choice to stderr
 bool tune_log()
@@ -719,6 +714,12 @@ bool tune_log()
     return on;
 }
 
+// times OR sweeps of each candidate on a scratch copy of g: first the
+// block order of the two-kernel sweep (passes of each divisor of the XCD's
+// timeslices), then, with that order, the wavefront slabs, then the
+// pipelined sweep's lags; a candidate must beat the incumbent by 3% (launch
+// count, noise), and a choice other than the default must beat it again in
+// alternate timings.  Never fails (the defaults on any error).
 GsweepTuning tune_gsweep(const GaugeSweepArgs &ga, int gauge, bool fp64, hipStream_t s)
 {
     const Geom &geo = ga.geo;

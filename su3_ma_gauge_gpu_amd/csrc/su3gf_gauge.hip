@@ -934,9 +934,8 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep_pipe(GaugeSweepArgs a)
 // (D, Im c, Re c), up and down link added, through LDS; wave 0 adds the four
 // in the order mu = 0..3, computes the update (drawing the site's random
 // numbers, where the update takes any) and hands it to the other waves
-// through LDS: a square root, a division and two inverse square roots per
-// subgroup that the other three waves no longer repeat.  Wave 0 also updates
-// and stores g(x).
+// through LDS, so that they do not repeat its square root, division and two
+// inverse square roots.  Wave 0 also updates and stores g(x).
 // theta (THETA = 1 before, 2 after the update) comes from the links as in the
 // link sweep's MAG check.
 template <typename A, int ALGO, int THETA, int LF>

@@ -457,6 +457,11 @@ def workload_gribov(args):
     streams = [_stream() for _ in range(args.copies)]
     line = gribov_line(args, streams)
     line["with_sa_graph"] = gribov_line(args, streams, graph=True)
+    for key, kw in (("one_launch", {}), ("one_launch_sa_graph", {"graph": True}),
+                    ("one_launch_all_sixteen", {"limit": 0})):
+        line[key] = gribov_one_launch_line(args, streams, **kw)
+        line[key]["same_best_gff_as_per_copy_launches"] = same_copies(line[key], line)
+    line["small_copies_annealing"] = {f"{L}^4": small_copies_line(args, streams, L) for L in (8, 12)}
     fp64 = gribov_fp64(args, streams[0])
     line["fp64_reference_algorithm"] = fp64
     line["speedup_vs_fp64"] = fp64["total_s"] / line["total_s"]
@@ -557,6 +562,90 @@ def gribov_line(args, streams=None, graph=False) -> dict:
     return line
 
 
+def gribov_one_launch_line(args, streams, graph=False, limit=None) -> dict:
+    """The gauge copies of gribov_line (same configurations, seeds and RNG
+    counters) with all sixteen annealed through one gf_batch_run_sa call with
+    GF_SA_ONE_LAUNCH: each half-sweep of a group of lattices is one launch
+    (groups of at most `limit` 64-site blocks per parity, the library's
+    default 8192 when None; 0: all sixteen in one launch), then OR-fixed
+    through one gf_batch_run_mixed call.  Every copy's result is that of
+    gribov_line, bit for bit."""
+    sizes = [16, 20, 24, 32]
+    lats, bufs, hs_all, vols = [], [], [], []
+    tuning_s, haar = 0.0, 10
+    for L in sizes:
+        dims = (L, L, L, L)
+        lat = gf.Lattice(dims)
+        hs = list(range(haar, haar + args.copies))
+        haar += args.copies
+        b = [gf.DeviceBuffer.from_numpy(gf.make_config(dims, 0.3, 1, h)) for h in hs]
+        tuning_s += pretune(lat, b[0], gf.LANDAU, [args.mode], streams[0], sa_mode=args.sa_mode)
+        lats += [lat] * args.copies
+        bufs += b
+        hs_all += hs
+        vols += [L ** 4] * args.copies
+    prev = gf.set_batch_launch_blocks(limit) if limit is not None else None
+    try:
+        _lib.call("gf_synchronize")
+        t0 = time.perf_counter()
+        sa = gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject,
+                             seeds=[1] * len(lats), counters=[(h - 10) * 100000 for h in hs_all], mode=args.sa_mode,
+                             max_concurrent=args.copies, streams=streams, graph=graph, one_launch=True)
+        t1 = time.perf_counter()
+        res = gf.batch_run_mixed(lats, bufs, gf.LANDAU, args.omega, args.switch, args.precision, args.maxiter,
+                                 mode=args.mode, check32=args.check32, max_concurrent=args.copies, streams=streams)
+        total_s = time.perf_counter() - t0
+    finally:
+        if prev is not None:
+            gf.set_batch_launch_blocks(prev)
+    per_size, total_su = {}, 0
+    for k, ((_, ntemps), r) in enumerate(zip(sa, res)):
+        total_su += (ntemps * (1 + 3) + r.sweeps) * vols[k]
+        key = f"{sizes[k // args.copies]}^4"
+        per_size.setdefault(key, {"best_gff": -1.0, "or_sweeps": []})
+        per_size[key]["best_gff"] = max(per_size[key]["best_gff"], r.gff)
+        per_size[key]["or_sweeps"].append(r.sweeps)
+    print(f"bench: gauge copies, one-launch annealing (limit {limit}, graph {graph}): {total_s:.2f} s "
+          f"(annealing {t1 - t0:.2f} s)", file=sys.stderr, flush=True)
+    return {"total_s": total_s, "sa_s": t1 - t0, "or_s": total_s - (t1 - t0),
+            "site_updates_per_s": total_su / total_s, "launch_block_limit": limit if limit is not None else 8192,
+            "sa_graph": graph, "per_size": per_size, "tuning_s": tuning_s}
+
+
+def small_copies_line(args, streams, L: int, n: int = 16) -> dict:
+    """n gauge copies of an L^4 lattice (Haar seeds 100..), annealed as in
+    gribov_line (args.sa_steps temperatures, 3 microcanonical steps each) once
+    with a launch per copy (four copies concurrently, each on its own stream)
+    and once with each half-sweep of all n copies in one launch
+    (GF_SA_ONE_LAUNCH): lattices this small are bound by launch overhead, not
+    by bytes.  Both runs must leave the same links."""
+    dims = (L, L, L, L)
+    lats = [gf.Lattice(dims)] * n
+    cfgs = [gf.make_config(dims, 0.3, 1, 100 + c) for c in range(n)]
+    kw = dict(seeds=[1] * n, counters=[c * 100000 for c in range(n)], mode=args.sa_mode, max_concurrent=4,
+              streams=streams[:4])
+    pretune(lats[0], gf.DeviceBuffer.from_numpy(cfgs[0]), gf.LANDAU, [], streams[0], sa_mode=args.sa_mode)
+    out = {}
+    for key, one in (("per_copy_launches_s", False), ("one_launch_s", True)):
+        bufs = [gf.DeviceBuffer.from_numpy(c) for c in cfgs]
+        _lib.call("gf_synchronize")
+        t0 = time.perf_counter()
+        gf.batch_run_sa(lats, bufs, gf.LANDAU, 1.45, 0.01, args.sa_steps, 3, args.reproject, one_launch=one, **kw)
+        out[key] = time.perf_counter() - t0
+        out[key + "_links"] = [b.download(np.float64, c.size) for b, c in zip(bufs, cfgs)]
+    same = all(np.array_equal(a, b) for a, b in zip(out.pop("per_copy_launches_s_links"), out.pop("one_launch_s_links")))
+    print(f"bench: {n} copies of {L}^4, annealing: {out['per_copy_launches_s']:.3f} s with a launch per copy, "
+          f"{out['one_launch_s']:.3f} s in one launch", file=sys.stderr, flush=True)
+    return {**out, "copies": n, "dims": list(dims), "identical_links": same,
+            "speedup_one_launch": out["per_copy_launches_s"] / out["one_launch_s"]}
+
+
+def same_copies(a: dict, b: dict) -> bool:
+    """Whether two gauge-copies runs ended on the same best functional for
+    every size (both runs take the same copies through the same updates)."""
+    return all(a["per_size"][k]["best_gff"] == b["per_size"][k]["best_gff"] for k in a["per_size"])
+
+
 def landau48_line(args) -> dict:
     """Landau gauge on a 48^4 lattice (sigma --sigma, seed 1, Haar seed 2):
     the working set (3.06 GB of FP64 links; 0.33 GB of 24-bit links and g in
@@ -649,12 +738,15 @@ def extra_workloads(args, streams) -> dict:
     for name, fn in (("coulomb_32x32x32x64", lambda: coulomb_line(one)),
                      ("gauge_copies", lambda: gribov_line(one, streams)),
                      ("gauge_copies_sa_graph", lambda: gribov_line(one, streams, graph=True)),
+                     ("gauge_copies_one_launch", lambda: gribov_one_launch_line(one, streams)),
                      ("mag_24x24x24x24", lambda: {**mag_line(one, 24, 200000), **mag_kernels(one, 48)}),
                      ("landau_48x48x48x48", lambda: landau48_line(one))):
         t0 = time.perf_counter()
         print(f"bench: workload {name}", file=sys.stderr, flush=True)
         out[name] = fn()
         out[name]["wall_s"] = time.perf_counter() - t0
+        if name == "gauge_copies_one_launch":
+            out[name]["same_best_gff_as_per_copy_launches"] = same_copies(out[name], out["gauge_copies"])
         _lib.call("gf_synchronize")
     return out
 

@@ -427,10 +427,29 @@ gf_status gf_batch_run_mixed(int n, const gf_lattice *lats, void *const *U, int 
                              gf_run_result *out);
 /* gf_run_sa for each problem, with seeds[k] and counters[k] (in/out: the
  * launch counter after the schedule); temperatures (optional) receives each
- * problem's number of temperatures; streams as for gf_batch_run_mixed. */
+ * problem's number of temperatures; streams as for gf_batch_run_mixed.
+ * mode | GF_SA_ONE_LAUNCH (modes 1 and 2; Landau, Coulomb on 4D lattices
+ * and U1xU1): instead of a launch per lattice, each half-sweep of several
+ * lattices is one launch whose blocks are the lattices' blocks one after
+ * another (up to 16 lattices of any sizes).  The lattices are packed into
+ * such groups largest first, each into the first group with room under the
+ * limit of gf_set_batch_launch_blocks; the groups anneal one after another on
+ * streams[0] (or a stream of the call's own), each group's schedule captured
+ * into one graph when GF_SA_GRAPH is set too.  Every problem's result and
+ * counter are those of gf_run_sa, bit for bit.  max_concurrent is not used.
+ * No reference counterpart. */
+#define GF_SA_ONE_LAUNCH 0x20
 gf_status gf_batch_run_sa(int n, const gf_lattice *lats, void *const *U, int gauge, float tmax, float tmin,
                           int steps, int micro, int reproject, int mode, const int *seeds, uint32_t *counters,
                           int *temperatures, int max_concurrent, void *const *streams);
+
+/* The most blocks of 64 sites of one parity that a GF_SA_ONE_LAUNCH launch
+ * gathers from several lattices (default 8192, the blocks of a 32^4 lattice,
+ * whose working set the 256 MB Infinity Cache holds in the annealing's
+ * 24-bit form; a lattice with more blocks forms a launch of its own); 0: no
+ * limit but the 16 lattices.  *previous (optional) receives the old limit.
+ * No reference counterpart. */
+gf_status gf_set_batch_launch_blocks(int blocks, int *previous);
 
 /* --------------------------------------------------- synthetic inputs
  * Host-side generator of the benchmark configurations (not a reference

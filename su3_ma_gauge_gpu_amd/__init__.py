@@ -48,6 +48,7 @@ from ._lib import (  # noqa: F401  (re-exported enums)
     U1XU1,
     UPDATE_DP,
     SA_GRAPH,
+    SA_ONE_LAUNCH,
     WAVEFRONT_PIPELINED,
     FILE_PLAIN,
     FILE_HEADERONLY,
@@ -65,7 +66,8 @@ from ._lib import (  # noqa: F401  (re-exported enums)
 __all__ = [
     "Lattice", "DeviceBuffer", "LandauKernelsSU3", "CoulombKernelsSU3", "MAGKernelsSU3",
     "U1xU1KernelsSU3", "CommonKernelsSU3", "GaugeFixingStats", "PhiloxWrapper", "GFError",
-    "make_config", "run_or", "run_fast", "run_fast_noalloc", "run_mixed", "run_sa", "batch_run_mixed", "batch_run_sa", "device_info",
+    "make_config", "run_or", "run_fast", "run_fast_noalloc", "run_mixed", "run_sa", "batch_run_mixed", "batch_run_sa", "set_batch_launch_blocks",
+    "device_info",
     "RunResult", "LinkFile",
 ]
 
@@ -467,13 +469,16 @@ def _batch_args(lattices, Us, streams, max_concurrent):
 
 def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: float = 0.01, steps: int = 1000,
                  micro: int = 3, reproject: int = 100, seeds=None, counters=None, mode: int = 0,
-                 max_concurrent: int = 4, streams=None, graph: bool = False) -> list[tuple[int, int]]:
+                 max_concurrent: int = 4, streams=None, graph: bool = False,
+                 one_launch: bool = False) -> list[tuple[int, int]]:
     """run_sa on independent 4D lattices in one call (gf_batch_run_sa in
     include/su3gf.h): small lattices concurrently, each on its own stream
     (``streams``: the caller's, one per concurrent problem, or None).
     ``seeds`` and ``counters`` give each problem's RNG seed and starting
-    launch counter.  Returns each problem's (next RNG counter, number of
-    temperatures)."""
+    launch counter.  ``one_launch`` (modes 1 and 2): each half-sweep of a
+    group of lattices is one launch (GF_SA_ONE_LAUNCH; groups limited by
+    set_batch_launch_blocks).  Returns each problem's (next RNG counter,
+    number of temperatures)."""
     n, lats, ptrs, strs = _batch_args(lattices, Us, streams, max_concurrent)
     seeds = list(seeds) if seeds is not None else [1] * n
     counters = list(counters) if counters is not None else [0] * n
@@ -483,8 +488,18 @@ def batch_run_sa(lattices, Us, gauge: int = LANDAU, tmax: float = 0.4, tmin: flo
     ctr = (C.c_uint32 * n)(*counters)
     temps = (C.c_int * n)()
     _lib.call("gf_batch_run_sa", n, lats, ptrs, gauge, C.c_float(tmax), C.c_float(tmin), int(steps), int(micro),
-              int(reproject), int(mode) | (SA_GRAPH if graph else 0), cs, ctr, temps, int(max_concurrent), strs)
+              int(reproject), int(mode) | (SA_GRAPH if graph else 0) | (SA_ONE_LAUNCH if one_launch else 0), cs, ctr,
+              temps, int(max_concurrent), strs)
     return [(int(ctr[k]), int(temps[k])) for k in range(n)]
+
+
+def set_batch_launch_blocks(blocks: int) -> int:
+    """Limit of the 64-site blocks of one parity that a one-launch batch
+    annealing launch gathers (gf_set_batch_launch_blocks; 0: no limit but 16
+    lattices).  Returns the previous limit."""
+    prev = C.c_int()
+    _lib.call("gf_set_batch_launch_blocks", int(blocks), C.byref(prev))
+    return prev.value
 
 
 def batch_run_mixed(lattices, Us, gauge: int = LANDAU, omega: float = 1.7, switch_precision: float = 1e-6,

@@ -15,6 +15,7 @@ LANDAU, COULOMB, MAG, U1XU1 = range(4)
 OR, MICRO, SA, SR, RANDOM = range(5)
 UPDATE_DP = 0x100  # OR'ed into an update rule: computed in double on FP32 links
 SA_GRAPH = 0x10  # OR'ed into an annealing mode: the schedule captured into one HIP graph
+SA_ONE_LAUNCH = 0x20  # OR'ed into a batch annealing mode: each half-sweep of several lattices in one launch
 F32, F64 = 0, 1
 R12, PATTERN18 = 12, 18
 MIXED_LINKS, MIXED_GAUGE32, MIXED_GAUGE64, MIXED_GAUGE24, MIXED_FP64 = 0, 1, 2, 3, 4
@@ -108,6 +109,7 @@ SIGNATURES = {
     "gf_pipelined_launches": (_i, [C.POINTER(C.c_longlong)]),
     "gf_set_pipelined_lag": (_i, [_i, C.POINTER(_i)]),
     "gf_set_block_order": (_i, [_i, C.POINTER(_i)]),
+    "gf_set_batch_launch_blocks": (_i, [_i, C.POINTER(_i)]),
     "gf_block_order_choice": (_i, [_pl, _i, _i, C.POINTER(_i)]),
     "gf_run_sa": (_i, [_pl, _vp, _vp, _i, _f, _f, _i, _i, _i, _i, _i, C.POINTER(C.c_uint32), C.POINTER(_i), _vp]),
     "gf_run_mixed": (_i, [_pl, _vp, _i, _f, _f, _f, _i, _i, _i, _i, _i, C.POINTER(gf_run_result), _vp]),

@@ -972,6 +972,93 @@ extern "C" gf_status gf_block_order_choice(const gf_lattice *lat, int gauge, int
 }
 
 // ------------------------------------------------------ simulated annealing
+namespace {
+// The reference apps' temperature schedules: heat(T) for each heatbath step,
+// project() when i % reproject == 0 for temperature i; *n receives the
+// number of temperatures.
+template <typename Heat, typename Project>
+gf_status sa_schedule(int gauge, float tmax, float tmin, int steps, int reproject, Heat &heat, Project &project,
+                      int *n)
+{
+    float temperature = tmax;
+    const float base = (tmax - tmin) / (float)steps;
+    *n = 0;
+    if (gauge == GF_LANDAU || gauge == GF_COULOMB) {
+        // fixed number of temperatures (LandauGaugeFixingSU3_4D.cu:205-240,
+        // CoulombGaugeFixingSU3_4D.cu:229-265)
+        for (int i = 0; i < steps; i++, (*n)++) {
+            GF_TRY(heat(temperature));
+            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
+            temperature -= base;
+        }
+    } else {
+        // down to tmin with steps 15 times finer in the critical window:
+        // U1xU1 2 <= T <= 2.7 (U1xU1GaugeFixingSU3_4D.cu:219-251), MAG
+        // 0.7 <= T <= 0.88 with five heatbath steps per temperature
+        // (MAGaugeFixingSU3_4D.cu:228-265); float T against double bounds
+        const double lo = gauge == GF_U1XU1 ? 2.0 : 0.7, hi = gauge == GF_U1XU1 ? 2.7 : 0.88;
+        const int per_t = gauge == GF_MAG ? 5 : 1;
+        int i = 0;
+        do {
+            const float step = (temperature <= hi && temperature >= lo) ? base / 15 : base;
+            for (int j = 0; j < per_t; j++) GF_TRY(heat(temperature));
+            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
+            temperature -= step;
+            i++;
+            (*n)++;
+        } while (temperature >= tmin);
+    }
+    return GF_OK;
+}
+
+// GF_SA_GRAPH: the schedule's launches are captured on the caller's stream,
+// or on a blocking stream of our own that follows the null stream, and
+// replayed once on it
+struct SaGraph {
+    hipStream_t own = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    bool capturing = false;
+    hipStream_t cs = nullptr;
+    SaGraph() = default;
+    SaGraph(const SaGraph &) = delete;
+    SaGraph &operator=(const SaGraph &) = delete;
+    ~SaGraph()
+    {
+        if (capturing) {
+            hipGraph_t dropped = nullptr;
+            (void)hipStreamEndCapture(cs, &dropped);
+            if (dropped) (void)hipGraphDestroy(dropped);
+        }
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        if (own) (void)hipStreamDestroy(own);
+    }
+    // starts the capture on *s (a stream of our own if *s is null), which becomes the capturing stream
+    gf_status begin(hipStream_t *s)
+    {
+        cs = *s;
+        if (!cs) {
+            HIP_TRY(hipStreamCreate(&own));
+            cs = own;
+        }
+        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        capturing = true;
+        *s = cs;
+        return GF_OK;
+    }
+    gf_status replay()
+    {
+        capturing = false;
+        HIP_TRY(hipStreamEndCapture(cs, &g));
+        HIP_TRY(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+        HIP_TRY(hipGraphLaunch(x, cs));
+        if (own) HIP_TRY(hipStreamSynchronize(own));
+        return GF_OK;
+    }
+};
+}  // namespace
+
 extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int gauge, float tmax, float tmin, int steps,
                                int micro, int reproject, int mode, int seed, uint32_t *counter, int *temperatures,
                                void *stream)
@@ -1058,73 +1145,11 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
             for (int par = 0; par < 2; par++) GF_TRY(gf_sweep(lat, U, dw, par, gauge, GF_MICRO, 0.f, 0, 0, 32, s));
         return GF_OK;
     };
-    // graph mode: the schedule's launches are captured on `cs` (the caller's
-    // stream, or a blocking stream of our own that follows the null stream)
-    // and replayed once on it
-    hipStream_t cs = s;
-    struct GraphRun {
-        hipStream_t own = nullptr;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t x = nullptr;
-        bool capturing = false;
-        hipStream_t cs = nullptr;
-        ~GraphRun()
-        {
-            if (capturing) {
-                hipGraph_t dropped = nullptr;
-                (void)hipStreamEndCapture(cs, &dropped);
-                if (dropped) (void)hipGraphDestroy(dropped);
-            }
-            if (x) (void)hipGraphExecDestroy(x);
-            if (g) (void)hipGraphDestroy(g);
-            if (own) (void)hipStreamDestroy(own);
-        }
-    } gr;
-    if (graph) {
-        if (!cs) {
-            HIP_TRY(hipStreamCreate(&gr.own));
-            cs = gr.own;
-        }
-        gr.cs = cs;
-        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        gr.capturing = true;
-        s = cs;  // every launch below is captured
-    }
-    float temperature = tmax;
-    const float base = (tmax - tmin) / (float)steps;
+    SaGraph gr;
+    if (graph) GF_TRY(gr.begin(&s));  // every launch below is captured
     int n = 0;
-    if (gauge == GF_LANDAU || gauge == GF_COULOMB) {
-        // fixed number of temperatures (LandauGaugeFixingSU3_4D.cu:205-240,
-        // CoulombGaugeFixingSU3_4D.cu:229-265)
-        for (int i = 0; i < steps; i++, n++) {
-            GF_TRY(heat(temperature));
-            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
-            temperature -= base;
-        }
-    } else {
-        // down to tmin with steps 15 times finer in the critical window:
-        // U1xU1 2 <= T <= 2.7 (U1xU1GaugeFixingSU3_4D.cu:219-251), MAG
-        // 0.7 <= T <= 0.88 with five heatbath steps per temperature
-        // (MAGaugeFixingSU3_4D.cu:228-265); float T against double bounds
-        const double lo = gauge == GF_U1XU1 ? 2.0 : 0.7, hi = gauge == GF_U1XU1 ? 2.7 : 0.88;
-        const int per_t = gauge == GF_MAG ? 5 : 1;
-        int i = 0;
-        do {
-            const float step = (temperature <= hi && temperature >= lo) ? base / 15 : base;
-            for (int j = 0; j < per_t; j++) GF_TRY(heat(temperature));
-            if (reproject > 0 && i % reproject == 0) GF_TRY(project());
-            temperature -= step;
-            i++;
-            n++;
-        } while (temperature >= tmin);
-    }
-    if (graph) {
-        gr.capturing = false;
-        HIP_TRY(hipStreamEndCapture(cs, &gr.g));
-        HIP_TRY(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
-        HIP_TRY(hipGraphLaunch(gr.x, cs));
-        if (gr.own) HIP_TRY(hipStreamSynchronize(gr.own));
-    }
+    GF_TRY(sa_schedule(gauge, tmax, tmin, steps, reproject, heat, project, &n));
+    if (graph) GF_TRY(gr.replay());
     if (gform) {
         // U <- G U G^dagger in FP64 (G the FP64 projection of g), third rows rebuilt
         HIP_TRY(launch_project_gauge(bg.as<float>(), V, s));
@@ -1135,6 +1160,158 @@ extern "C" gf_status gf_run_sa(const gf_lattice *lat, void *U, void *Udw, int ga
         GF_TRY(pipe.check(s));
     }
     if (temperatures) *temperatures = n;
+    return GF_OK;
+}
+
+// ------------------------------------------- annealing of a batch in one launch
+namespace {
+std::mutex g_batch_mu;
+int g_batch_launch_blocks = 8192;  // gf_set_batch_launch_blocks
+
+// Modes 1, 2 of gf_run_sa on the lattices idx[0..n) (n <= kMultiMax) with
+// every half-sweep of all of them in one launch (launch_gsweep_multi); the
+// lattices have been checked.
+gf_status run_sa_multi(const int *idx, int n, const gf_lattice *lats, void *const *U, int gauge, float tmax, float tmin,
+                       int steps, int micro, int mode, bool graph, const int *seeds, uint32_t *counters,
+                       int *temperatures, hipStream_t s)
+{
+    Scratch bU[kMultiMax], bg[kMultiMax];
+    MultiSweepArgs m{};
+    m.n = n;
+    for (int j = 0; j < n; j++) {
+        const int k = idx[j];
+        const gf_lattice *lat = &lats[k];
+        int64_t V;
+        GF_TRY(check_lattice(lat, &V));
+        gf_lattice l32 = *lat;
+        l32.dtype = GF_F32;
+        l32.layout = GF_LAYOUT_R12;
+        HIP_TRY(bU[j].alloc(mode == 2 ? sizeof(uint32_t) * links24_words(V) : gf_lattice_bytes(&l32), s));
+        HIP_TRY(bg[j].alloc(sizeof(float) * gauge_floats(V), s));
+        HIP_TRY(launch_gauge_identity(bg[j].as<float>(), V, s));
+        if (mode == 2) HIP_TRY(launch_pack24(lat->dtype, lat->layout, U[k], bU[j].as<uint32_t>(), V, s));
+        else HIP_TRY(launch_convert(lat->dtype, lat->layout, U[k], GF_F32, 12, bU[j].p, V, 0, s));
+        m.item[j] = MultiItem{bU[j].as<const float>(), bg[j].as<float>(), make_geom(lat->dims, V), (uint32_t)seeds[k],
+                              counters[k]};
+    }
+    uint32_t add = 0;  // every lattice's counter advances by 2 per heatbath step, as in gf_run_sa
+    auto pair = [&](int algo) -> hipError_t {
+        for (int par = 0; par < 2; par++) {
+            m.parity = par;
+            const hipError_t e = launch_gsweep_multi(gauge, algo, mode == 2, m, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    auto heat = [&](float T) -> gf_status {
+        m.param = T;
+        m.counter_add = add;
+        HIP_TRY(pair(GF_SA));
+        add += 2;
+        for (int k = 0; k < micro; k++) HIP_TRY(pair(GF_MICRO));
+        return GF_OK;
+    };
+    auto project = []() -> gf_status { return GF_OK; };  // the sweep keeps g unitary itself
+    SaGraph gr;
+    if (graph) GF_TRY(gr.begin(&s));
+    int nt = 0;
+    GF_TRY(sa_schedule(gauge, tmax, tmin, steps, 0, heat, project, &nt));
+    if (graph) GF_TRY(gr.replay());
+    for (int j = 0; j < n; j++) {
+        const int k = idx[j];
+        const gf_lattice *lat = &lats[k];
+        const Geom &geo = m.item[j].geo;
+        HIP_TRY(launch_project_gauge(bg[j].as<float>(), geo.V, s));
+        HIP_TRY(launch_gauge_apply((const double *)U[k], lat->layout, bg[j].as<const float>(), U[k], GF_F64,
+                                   lat->layout, geo, s));
+        GF_TRY(gf_project(lat, U[k], s));
+        counters[k] += add;
+        if (temperatures) temperatures[k] = nt;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return GF_OK;
+}
+}  // namespace
+
+extern "C" gf_status gf_set_batch_launch_blocks(int blocks, int *previous)
+{
+    if (blocks < 0) return fail(GF_ERR_BAD_ARG, "blocks per batch launch %d (0: no limit)", blocks);
+    std::lock_guard<std::mutex> lock(g_batch_mu);
+    if (previous) *previous = g_batch_launch_blocks;
+    g_batch_launch_blocks = blocks;
+    return GF_OK;
+}
+
+// gf_batch_run_sa with GF_SA_ONE_LAUNCH: the lattices are packed into groups
+// (largest first, each into the first group with room: at most kMultiMax
+// lattices and g_batch_launch_blocks blocks of a parity, unless the lattice
+// alone has more) and the groups anneal one after another on s.
+gf_status su3gf::batch_sa_one_launch(int n, const gf_lattice *lats, void *const *U, int gauge, float tmax, float tmin,
+                                     int steps, int micro, int mode, const int *seeds, uint32_t *counters,
+                                     int *temperatures, void *stream)
+{
+    const bool graph = (mode & GF_SA_GRAPH) != 0;
+    mode &= ~(GF_SA_GRAPH | GF_SA_ONE_LAUNCH);
+    if (mode != 1 && mode != 2) return fail(GF_ERR_BAD_ARG, "one-launch annealing takes modes 1 and 2, not %d", mode);
+    if (gauge != GF_LANDAU && gauge != GF_COULOMB && gauge != GF_U1XU1)
+        return fail(GF_ERR_BAD_ARG, "one-launch annealing: Landau, Coulomb and U1xU1 (gauge %d)", gauge);
+    if (steps < 0 || micro < 0) return fail(GF_ERR_BAD_ARG, "negative step count");
+    std::vector<int64_t> blocks(n);
+    for (int k = 0; k < n; k++) {
+        int64_t V;
+        if (check_lattice(&lats[k], &V) != GF_OK || check_gauge_sweep_volume(V) != GF_OK) {
+            const std::string msg = gf_last_error();
+            return fail(GF_ERR_BAD_SHAPE, "batch problem %d: %s", k, msg.c_str());
+        }
+        if (lats[k].dtype != GF_F64) return fail(GF_ERR_BAD_ARG, "batch problem %d: modes 1 and 2 take an FP64 configuration", k);
+        if (gauge == GF_COULOMB && lats[k].dims[0] == 1)
+            return fail(GF_ERR_BAD_ARG, "batch problem %d: modes 1 and 2 anneal every timeslice of a 4D lattice at once", k);
+        blocks[k] = sweep_grid(make_geom(lats[k].dims, V));
+    }
+    int cap;
+    {
+        std::lock_guard<std::mutex> lock(g_batch_mu);
+        cap = g_batch_launch_blocks;
+    }
+    std::vector<int> order(n);
+    for (int k = 0; k < n; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return blocks[a] > blocks[b]; });
+    std::vector<std::vector<int>> groups;
+    std::vector<int64_t> load;
+    for (int k : order) {
+        size_t gi = 0;
+        while (gi < groups.size() &&
+               (groups[gi].size() >= (size_t)kMultiMax || (cap > 0 && load[gi] + blocks[k] > cap)))
+            gi++;
+        if (gi == groups.size()) {
+            groups.emplace_back();
+            load.push_back(0);
+        }
+        groups[gi].push_back(k);
+        load[gi] += blocks[k];
+    }
+    struct OwnStream {
+        hipStream_t s = nullptr;
+        ~OwnStream()
+        {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } own;
+    hipStream_t s = S(stream);
+    if (!s) {
+        HIP_TRY(hipStreamCreateWithFlags(&own.s, hipStreamNonBlocking));
+        s = own.s;
+    }
+    for (const auto &grp : groups) {
+        std::vector<int> idx(grp.begin(), grp.end());
+        std::sort(idx.begin(), idx.end());  // problem order within a launch
+        const gf_status st = run_sa_multi(idx.data(), (int)idx.size(), lats, U, gauge, tmax, tmin, steps, micro, mode,
+                                          graph, seeds, counters, temperatures, s);
+        if (st != GF_OK) {
+            const std::string msg = gf_last_error();
+            return fail(st, "batch launch group of problem %d: %s", idx[0], msg.c_str());
+        }
+    }
     return GF_OK;
 }
 

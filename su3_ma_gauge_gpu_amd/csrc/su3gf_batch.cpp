@@ -146,6 +146,9 @@ extern "C" gf_status gf_batch_run_sa(int n, const gf_lattice *lats, void *const 
     if (st != GF_OK) return st;
     if (n == 0) return GF_OK;
     if (!seeds || !counters) return su3gf::set_error(GF_ERR_BAD_ARG, "null argument");
+    if (mode & GF_SA_ONE_LAUNCH)
+        return su3gf::batch_sa_one_launch(n, lats, U, gauge, tmax, tmin, steps, micro, mode, seeds, counters,
+                                          temperatures, streams ? streams[0] : nullptr);
     return run_batch(n, lats, max_concurrent, streams, [&](int k, void *s) {
         return gf_run_sa(&lats[k], U[k], nullptr, gauge, tmax, tmin, steps, micro, reproject, mode, seeds[k],
                          &counters[k], temperatures ? &temperatures[k] : nullptr, s);

@@ -247,7 +247,7 @@ template <int LF> __device__ __forceinline__ Field links_field(const float *U0, 
 }
 
 // g: FP32 tiles, or FP64 tiles with FP64 links (LF = 2)
-template <int LF> __device__ __forceinline__ Field g_field(const GaugeSweepArgs &a)
+template <int LF, typename Args> __device__ __forceinline__ Field g_field(const Args &a)
 {
     if (LF == 2) return make_tiled_field64((const double *)a.g, a.geo.V);
     return make_tiled_field(a.g, a.geo.V, 12);
@@ -520,8 +520,8 @@ __device__ __forceinline__ void gauge_update(A q[4], float param, Philox &rng)
 // X = U0_mu(x) g(y)^dagger and Z = g(z) U0_mu(z), y = x+mu, z = x-mu, so that
 // S_mu = X^dagger + Z.  coh (uniform): the neighbours' g through loads
 // coherent at device scope.
-template <typename A, int LF>
-__device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const Field &fg, int mu, const int c[4],
+template <typename A, int LF, typename Args>
+__device__ __forceinline__ void direction_terms(const Args &a, const Field &fg, int mu, const int c[4],
                                                 int par, uint32_t site_off, M3<A> &X, M3<A> &Z, bool coh = false)
 {
     M3<A> T, W;
@@ -544,8 +544,8 @@ __device__ __forceinline__ void direction_terms(const GaugeSweepArgs &a, const F
 // reads it again for the theta of the updated state) and g(x) rows 0,1 in
 // Gx: the three subgroup updates of g(x), its store, and the fused-check
 // partial sums of the 64-site block blk.
-template <typename A, int G, int ALGO, int THETA, int LF, typename SumS>
-__device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &fg, uint32_t site_off, bool active,
+template <typename A, int G, int ALGO, int THETA, int LF, typename SumS, typename Args>
+__device__ __forceinline__ void site_tail(const Args &a, const Field &fg, uint32_t site_off, bool active,
                                           int64_t blk, int lane, const SumS &sum_s, M3<A> Gx, Philox &rng,
                                           bool sc1_store = false)
 {
@@ -626,7 +626,7 @@ __device__ __forceinline__ void site_tail(const GaugeSweepArgs &a, const Field &
 // Philox generator of a site (SA, random transformation): key word 0 = global
 // id of the reference's (mu = 0, up) thread of this site in its V/2/NSB x
 // 8*NSB launch (PhiloxWrapper.hxx:93-110)
-__device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, int parity, Philox &rng)
+template <typename Args> __device__ __forceinline__ void site_rng(const Args &a, int64_t p, int parity, Philox &rng)
 {
     const uint32_t tid = (uint32_t)((p / a.nsb) * 8 * a.nsb + p % a.nsb);
     philox_init(rng, tid, a.seed, a.wf && parity ? a.counter_odd : a.counter);
@@ -635,25 +635,17 @@ __device__ __forceinline__ void site_rng(const GaugeSweepArgs &a, int64_t p, int
 // Work mapping 0: a block covers 64 sites with one wave per direction of S
 // (mu = 0..3; Coulomb mu = 1..3), each forming the direction-mu terms of its
 // 64 sites (four 12-real loads per lane, all coalesced).  The terms go through
-// LDS; wave 0 adds them in the order of mu and runs the tail.
-template <typename A, int G, int ALGO, int THETA, int LF>
-__global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
+// LDS; wave 0 adds them in the order of mu and runs the tail.  gsweep_block
+// sweeps the sites of parity `parity` in the 64-site block blk of the lattice
+// of a; k_gsweep and k_gsweep_multi choose the block.
+template <typename A, int G, int ALGO, int THETA, int LF, typename Args>
+__device__ __forceinline__ void gsweep_block(const Args &a, int64_t blk, int parity)
 {
     constexpr int kW = gsweep_waves<G>();
     __shared__ A lds[18 * kW * 64];
-    static_assert(LF != 2 || sizeof(A) == 8, "FP64 storage takes FP64 arithmetic");
-    if (a.state && a.state->done) return;  // converged: uniform early exit
-    if (THETA == 3 && !a.state->near) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mu = wave + (4 - kW);  // one direction per wave
     const int lane = threadIdx.x & 63;
-    int parity = a.parity;
-    int64_t blk;
-    if (a.wf) {
-        if (!wave_block(a, blk, parity)) return;
-    } else {
-        blk = logical_block(a);
-    }
     bool active;
     const int64_t p = block_site(a.geo, blk, lane, active);
     const int64_t site = (active ? p : 0) + (parity ? a.geo.Vh : 0);
@@ -705,6 +697,56 @@ __global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
             }
     };
     site_tail<A, G, ALGO, THETA, LF>(a, fg, site_off, active, blk, lane, sum_s, Gx, rng);
+}
+
+template <typename A, int G, int ALGO, int THETA, int LF>
+__global__ __launch_bounds__(kGBlock) void k_gsweep(GaugeSweepArgs a)
+{
+    static_assert(LF != 2 || sizeof(A) == 8, "FP64 storage takes FP64 arithmetic");
+    if (a.state && a.state->done) return;  // converged: uniform early exit
+    if (THETA == 3 && !a.state->near) return;
+    int parity = a.parity;
+    int64_t blk;
+    if (a.wf) {
+        if (!wave_block(a, blk, parity)) return;
+    } else {
+        blk = logical_block(a);
+    }
+    gsweep_block<A, G, ALGO, THETA, LF>(a, blk, parity);
+}
+
+// Several lattices in one launch (the annealing of a batch): their blocks of
+// one parity are concatenated in lattice order, and the XCDs take contiguous
+// eighths of that sequence (block b runs on XCD b % 8), as remap = 1 does for
+// one lattice.  Lattice k's sites are those of the single-lattice sweep with
+// its own seed and RNG counter item[k].counter + counter_add + parity: the
+// same updates and random numbers.  The lattice's fields as gsweep_block reads
+// them, its geometry by reference into the kernel argument (indexed by
+// direction, which a local copy would have to keep in scratch memory).
+struct MultiView {
+    const float *U0;
+    float *g;
+    const Geom &geo;
+    float param;
+    uint32_t seed, counter;
+    int64_t nsb;
+    int wf;
+    uint32_t counter_odd;
+    double *partial;
+};
+
+template <typename A, int G, int ALGO, int LF>
+__global__ __launch_bounds__(kGBlock) void k_gsweep_multi(MultiSweepArgs m)
+{
+    const uint32_t b = blockIdx.x;
+    const int32_t L = (int32_t)((b & 7u) * (uint32_t)m.per_xcd + (b >> 3));
+    if (L >= m.first[m.n]) return;  // padding of the last XCD's range
+    int k = 0;
+    while (k + 1 < m.n && m.first[k + 1] <= L) k++;
+    const MultiItem &it = m.item[k];
+    const MultiView a{it.U0, it.g, it.geo, m.param, it.seed, it.counter + m.counter_add + (uint32_t)m.parity,
+                      32, 0, 0, nullptr};
+    gsweep_block<A, G, ALGO, 0, LF>(a, L - m.first[k], m.parity);
 }
 
 // ------------------------------------------------------ pipelined sweep
@@ -1386,6 +1428,45 @@ hipError_t launch_gsweep_pipe(int gauge, int algo, bool fp64_arith, const GaugeS
                                                : launch_pipe_lf<GF_U1XU1>(algo, fp64_arith, b, s);
     if (e == hipSuccess) g_pipe_launches++;
     return e;
+}
+
+}  // namespace su3gf
+
+namespace su3gf {
+namespace {
+template <int G, int LF> hipError_t launch_multi_algo(int algo, const MultiSweepArgs &m, unsigned grid, hipStream_t s)
+{
+    const dim3 block(64 * gsweep_waves<G>());
+    if (algo == GF_SA) hipLaunchKernelGGL((k_gsweep_multi<float, G, GF_SA, LF>), dim3(grid), block, 0, s, m);
+    else if (algo == GF_MICRO) hipLaunchKernelGGL((k_gsweep_multi<float, G, GF_MICRO, LF>), dim3(grid), block, 0, s, m);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int G> hipError_t launch_multi_g(int algo, int links24, const MultiSweepArgs &m, unsigned grid, hipStream_t s)
+{
+    return links24 ? launch_multi_algo<G, 1>(algo, m, grid, s) : launch_multi_algo<G, 0>(algo, m, grid, s);
+}
+}  // namespace
+
+hipError_t launch_gsweep_multi(int gauge, int algo, int links24, MultiSweepArgs &m, hipStream_t s)
+{
+    if (m.n < 1 || m.n > kMultiMax || (m.parity & ~1)) return hipErrorInvalidValue;
+    int64_t total = 0;
+    for (int k = 0; k < m.n; k++) {
+        const Geom &g = m.item[k].geo;
+        if (!gauge_sweep_fits(g.V) || g.slice_bps != 0 || !m.item[k].U0 || !m.item[k].g) return hipErrorInvalidValue;
+        m.first[k] = (int32_t)total;
+        total += sweep_grid(g);
+        if (total >= (int64_t(1) << 30)) return hipErrorInvalidValue;
+    }
+    m.first[m.n] = (int32_t)total;
+    m.per_xcd = (int32_t)((total + 7) / 8);
+    const unsigned grid = 8u * (unsigned)m.per_xcd;
+    if (gauge == GF_LANDAU) return launch_multi_g<GF_LANDAU>(algo, links24, m, grid, s);
+    if (gauge == GF_COULOMB) return launch_multi_g<GF_COULOMB>(algo, links24, m, grid, s);
+    if (gauge == GF_U1XU1) return launch_multi_g<GF_U1XU1>(algo, links24, m, grid, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace su3gf

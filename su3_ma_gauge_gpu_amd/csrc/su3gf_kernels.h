@@ -151,6 +151,31 @@ constexpr int kPipelinedSlab = GF_WAVEFRONT_PIPELINED;
 // a.pipe.  Returns hipErrorNotSupported where it does not apply (the caller
 // then launches the two kernels).
 hipError_t launch_gsweep_pipe(int gauge, int algo, bool fp64_arith, const GaugeSweepArgs &a, hipStream_t s);
+// Up to kMultiMax lattices swept in one launch (launch_gsweep_multi): each
+// lattice's read-only links U0 (FP32 rows 0,1 or the 24-bit words) and its g,
+// with the geometry and the RNG key of its single-lattice sweep.
+constexpr int kMultiMax = 16;
+struct MultiItem {
+    const float *U0;
+    float *g;
+    Geom geo;
+    uint32_t seed, counter;  // Philox key and the counter before counter_add
+};
+struct MultiSweepArgs {
+    MultiItem item[kMultiMax];
+    int32_t first[kMultiMax + 1];  // first block of each lattice in the concatenation; first[n] = total
+    int n;
+    int parity;
+    float param;           // SA temperature (shared by the lattices)
+    uint32_t counter_add;  // added to every item's counter (then + parity)
+    int32_t per_xcd;       // set by the launcher: blocks per XCD, ceil(total / 8)
+};
+// One parity of every lattice of m in one launch, each block running the
+// two-kernel sweep's code on the block of its lattice: the results are those
+// of launch_gsweep on each lattice with counter item.counter + counter_add +
+// parity.  Landau, Coulomb (all timeslices at once) and U1xU1; SA and
+// microcanonical; FP32 arithmetic on FP32 (links24 = 0) or 24-bit links.
+hipError_t launch_gsweep_multi(int gauge, int algo, int links24, MultiSweepArgs &m, hipStream_t s);
 // whether the pipelined sweep applies to a lattice and gauge (the launcher's
 // shape conditions), so that callers allocate its words only then
 bool gsweep_pipe_applies(const Geom &geo, int gauge, bool links64);

@@ -121,6 +121,27 @@ def test_batch_calls_check_every_problem_on_the_host(gf):
     assert lib.gf_batch_run_sa(2, lats, ptrs, 0, 1.0, 0.1, 10, 3, 100, 1, seeds, ctr, None, 4, None) == 1
     assert lib.gf_batch_run_sa(2, lats_ok, ptrs, 0, 1.0, 0.1, 10, 3, 100, 1, None, ctr, None, 4, None) == 2
     assert lib.gf_batch_run_sa(0, None, None, 0, 1.0, 0.1, 10, 3, 100, 1, None, None, None, 4, None) == 0
+    # GF_SA_ONE_LAUNCH: modes 1 and 2 on FP64 lattices, Landau, Coulomb on a
+    # 4D lattice and U1xU1, checked before anything runs
+    one = gf.SA_ONE_LAUNCH
+    assert lib.gf_batch_run_sa(2, lats_ok, ptrs, 0, 1.0, 0.1, 10, 3, 100, one | 0, seeds, ctr, None, 4, None) == 2
+    assert "modes 1 and 2" in gf._lib.last_error()
+    assert lib.gf_batch_run_sa(2, lats_ok, ptrs, gf.MAG, 1.0, 0.1, 10, 3, 100, one | 2, seeds, ctr, None, 4,
+                               None) == 2
+    f32 = gf.Lattice((4, 4, 4, 4), dtype=gf.F32).struct
+    assert lib.gf_batch_run_sa(2, (gf._lib.gf_lattice * 2)(good, f32), ptrs, 0, 1.0, 0.1, 10, 3, 100, one | 2, seeds,
+                               ctr, None, 4, None) == 2
+    assert "batch problem 1" in gf._lib.last_error()
+    slice3 = _lat(gf, (1, 4, 4, 4))
+    assert lib.gf_batch_run_sa(2, (gf._lib.gf_lattice * 2)(slice3, good), ptrs, gf.COULOMB, 1.0, 0.1, 10, 3, 100,
+                               one | 2, seeds, ctr, None, 4, None) == 2
+    assert "batch problem 0" in gf._lib.last_error()
+    assert lib.gf_batch_run_sa(2, lats, ptrs, 0, 1.0, 0.1, 10, 3, 100, one | 2, seeds, ctr, None, 4, None) == 1
+    assert list(ctr) == [0, 0]
+    prev = C.c_int()
+    assert lib.gf_set_batch_launch_blocks(-1, C.byref(prev)) == 2
+    assert lib.gf_set_batch_launch_blocks(4096, C.byref(prev)) == 0 and prev.value == 8192
+    assert gf.set_batch_launch_blocks(prev.value) == 4096
     # the caller's streams: one per concurrent problem, none of them null
     short = (C.c_void_p * 4)(0x10, 0x20, None, 0x40)
     assert lib.gf_batch_run_mixed(2, lats_ok, ptrs, 0, 1.7, 1e-6, 1e-12, 10, 3, 50, 10, 100, 4, short, out) == 2

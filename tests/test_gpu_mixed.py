@@ -116,6 +116,39 @@ def test_batch_runs_match_single_runs(gf, gpu, own_streams):
         gf._lib.call("gf_stream_destroy", s)
 
 
+@pytest.mark.parametrize("gauge,mode,graph,limit", [(LANDAU, 2, False, 8192), (LANDAU, 1, False, 100),
+                                                    (LANDAU, 2, True, 0), (COULOMB, 2, False, 8192),
+                                                    (U1XU1, 1, False, 8192), (U1XU1, 2, True, 50)])
+def test_one_launch_batch_annealing_matches_single_runs(gf, gpu, gauge, mode, graph, limit):
+    """GF_SA_ONE_LAUNCH: lattices of several shapes (odd block counts, a
+    last block with idle lanes, extents 4 to 12) annealed with each half-sweep
+    of a group of them in one launch -- one group, several groups under a
+    small block limit (a lattice above the limit alone), no limit, and the
+    groups' schedules captured into graphs -- give every problem the links,
+    RNG counter and temperature count of gf_run_sa on that lattice alone, bit
+    for bit.  The problems have their own seeds and starting counters."""
+    shapes = [(8, 8, 8, 8), (4, 6, 4, 8), (12, 12, 12, 12), (8, 8, 8, 8), (6, 6, 6, 10), (4, 4, 4, 4)]
+    sigma = 0.3
+    cfgs = [gf.make_config(d, sigma, 1 + k, 10 + k) for k, d in enumerate(shapes)]
+    lats = [gf.Lattice(d) for d in shapes]
+    seeds = [3, 4, 5, 3, 7, 8]
+    counters = [0, 1000, 2000, 2**32 - 6, 3, 77]  # one counter wraps past 2^32
+    kw = dict(tmax=2.8, tmin=1.9, steps=10) if gauge == U1XU1 else dict(tmax=1.45, tmin=0.01, steps=12)
+    prev = gf.set_batch_launch_blocks(limit)
+    try:
+        bufs = [gf.DeviceBuffer.from_numpy(c) for c in cfgs]
+        out = gf.batch_run_sa(lats, bufs, gauge, micro=2, reproject=5, seeds=seeds, counters=counters, mode=mode,
+                              graph=graph, one_launch=True, **kw)
+    finally:
+        gf.set_batch_launch_blocks(prev)
+    for k, dims in enumerate(shapes):
+        d = gf.DeviceBuffer.from_numpy(cfgs[k])
+        ref = gf.run_sa(lats[k], d, gauge, micro=2, reproject=5, seed=seeds[k], counter=counters[k], mode=mode, **kw)
+        gf.synchronize()
+        assert out[k] == ref, (k, out[k], ref)
+        assert np.array_equal(bufs[k].download(np.float64, cfgs[k].size), d.download(np.float64, cfgs[k].size)), k
+
+
 def test_check_count_when_fp32_phase_takes_every_sweep(gf, gpu):
     """A mixed run whose FP32 phase uses up maxiter leaves the FP64 phase no
     sweeps; the reported checks are the FP32 phase's alone.  With check32 = 5

@@ -79,9 +79,11 @@ def parse():
     ap.add_argument("--mag-runs", default="both", choices=["both", "mixed", "fp64"],
                     help="--workload mag: the mixed and the all-FP64 run, or one of them (each takes minutes at 48^4)")
     ap.add_argument("--workload", default="landau",
-                    choices=["landau", "coulomb", "gribov", "mag"],
+                    choices=["landau", "coulomb", "gribov", "mag", "small-copies"],
                     help="landau: the 32^4 headline run (default); coulomb: 32^3x64 all slices at once; "
-                         "gribov: SA + OR gauge copies of 16^4..32^4; mag: 48^4 MAG then U1xU1")
+                         "gribov: SA + OR gauge copies of 16^4..32^4; mag: 48^4 MAG then U1xU1; "
+                         "small-copies: annealing of 16 copies each of 8^4 and 12^4, a launch per copy "
+                         "against one launch for all")
     return ap.parse_args()
 
 
@@ -468,6 +470,13 @@ def workload_gribov(args):
     print(json.dumps(line), flush=True)
 
 
+def workload_small_copies(args):
+    streams = [_stream() for _ in range(4)]
+    out = {f"{L}^4": small_copies_line(args, streams, L) for L in (8, 12)}
+    print(json.dumps({"workload": "small-copies-annealing", "sa_steps": args.sa_steps, "sa_mode": args.sa_mode,
+                      "results": out}), flush=True)
+
+
 def gribov_fp64(args, stream) -> dict:
     """The gauge copies of gribov_line as the reference apps run them, in FP64
     (the annealing on FP64 link sweeps, run_sa mode 0, then the FP64 OR of
@@ -758,7 +767,8 @@ def main():
               "per GPU and has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     if args.workload != "landau":
-        {"coulomb": workload_coulomb, "mag": workload_mag, "gribov": workload_gribov}[args.workload](args)
+        {"coulomb": workload_coulomb, "mag": workload_mag, "gribov": workload_gribov,
+         "small-copies": workload_small_copies}[args.workload](args)
         return
     # the gauge copies' streams, created before the first device allocation
     # (streams created after it have run the concurrent copies up to 35% slower)

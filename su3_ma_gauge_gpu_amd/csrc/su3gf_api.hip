@@ -4,6 +4,7 @@
 // reduction bookkeeping of GaugeFixingStats and the OR / mixed-precision
 // drivers.  Every HIP call is checked; failures come back as gf_status with
 // a message retrievable through gf_last_error().
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -1408,10 +1409,18 @@ int check_segments(const int dims[4], int gauge) { return gauge == GF_COULOMB &&
 // sweeps throughout.  coarse_start: the phase also checks coarsely before its
 // first read (a phase that starts far from tol) and turns fine for good at the
 // first theta near tol; otherwise it starts fine and follows the theta read
-// after each chunk.
+// after each chunk, and also by its rate: once two reads at least kRateSpan
+// sweeps apart show theta falling, the phase stays coarse while theta falling
+// at that rate would take more than kFarSweeps sweeps (five chunks) to reach
+// tol, however close to tol it is.  A phase that converges slowly (MAG's FP64
+// phase at large volume gains a factor of 10 in some 10^4 sweeps) thus checks
+// every sweep only near its end; theta would have to fall five times faster
+// than over the last kRateSpan sweeps to cross tol before the next read.
 constexpr double kCoarseFactor = 1e3;
 constexpr double kNearFactor = 2.0;
 constexpr double kHostNearFactor = 1e3;
+constexpr int kRateSpan = 2 * kChunk;
+constexpr int kFarSweeps = 5 * kChunk;
 
 // plain(it): enqueue all of sweep `it` (no check involved) at once and return
 // hipSuccess, or hipErrorNotSupported to have its two parity kernels launched
@@ -1433,6 +1442,8 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
     bool pending = false;  // a check of the state at the end of sweep it-1 is due
     bool coarse = coarse_every > check_every && coarse_start;
     bool host_near = exact_end;  // until the first read of this phase's state
+    int rate_sweep = -1;         // the rate rule: sweep and log theta of an earlier read check,
+    double rate_log = 0, rate = 0;  // and the fall of log theta per sweep since the one before
     auto check = [&](int mode, int sweep) -> gf_status {
         const hipError_t ec = launch_fused_check(fc.st, fc.even(), fc.odd(), nblocks, nseg, sweep, mode, s);
         if (ec != hipSuccess) return hip_fail(ec, "check kernel launch");
@@ -1472,9 +1483,20 @@ gf_status fused_loop(const int dims[4], int64_t V, int gauge, double tol, int ma
         GF_TRY(fc.read(s));
         done = fc.host->done != 0;
         if (coarse_every > check_every && fc.host->checks > 0) {
-            const bool far = fc.host->theta > kCoarseFactor * tol;
-            if (!coarse_start) coarse = far;  // coarse only while a theta the host has read is far
-            else if (!far) coarse = false;    // coarse until the first theta near tol
+            const double th = fc.host->theta;
+            bool far = th > kCoarseFactor * tol;
+            if (!coarse_start) {
+                const int cs = fc.host->checked_sweep;
+                if (th > 0 && (rate_sweep < 0 || cs - rate_sweep >= kRateSpan)) {
+                    if (rate_sweep >= 0) rate = (rate_log - std::log(th)) / (cs - rate_sweep);
+                    rate_sweep = cs;
+                    rate_log = std::log(th);
+                }
+                if (rate > 0 && th > tol) far = std::log(th / tol) > rate * kFarSweeps;
+                coarse = far;  // coarse only while a theta the host has read is far
+            } else if (!far) {
+                coarse = false;  // coarse until the first theta near tol
+            }
         }
         if (exact_end && fc.host->checks > 0) host_near = fc.host->near || !(fc.host->theta > kHostNearFactor * tol);
     }
@@ -1949,9 +1971,9 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     const bool near = reached_switch && switch_precision < kNearFactor * precision;
     // MAG's checks cost a third of a sweep in this phase (its theta is formed
     // from the links in every wave); far from the target it converges by a few
-    // percent per hundred sweeps, so it checks every 5 sweeps while a theta
-    // the host has read is above kCoarseFactor * precision
-    const int coarse64 = gauge == GF_MAG ? 5 : 0;
+    // percent per hundred sweeps, so it checks every 16 sweeps while a theta
+    // the host has read is far from the precision (fused_loop's rules)
+    const int coarse64 = gauge == GF_MAG ? 16 : 0;
     if (gauge64) {
         // FP64 sweeps of a second gauge transformation g64 (from 1) on the
         // links G U0 G^dagger, read only: the same OR updates as sweeping the

@@ -76,8 +76,9 @@ def parse():
     ap.add_argument("--mag-precision", type=float, default=1e-12)
     ap.add_argument("--maxiter-mag", type=int, default=600000)
     ap.add_argument("--u1xu1-sweeps", type=int, default=1000)
-    ap.add_argument("--mag-runs", default="both", choices=["both", "mixed", "fp64"],
-                    help="--workload mag: the mixed and the all-FP64 run, or one of them (each takes minutes at 48^4)")
+    ap.add_argument("--mag-runs", default="both", choices=["both", "mixed", "fp64", "fp64-gauge"],
+                    help="--workload mag: the mixed and the all-FP64 run, or one of them (each takes minutes at 48^4); "
+                         "fp64-gauge: all-FP64 on the FP64 gauge-transformation sweep")
     ap.add_argument("--workload", default="landau",
                     choices=["landau", "coulomb", "gribov", "mag", "small-copies"],
                     help="landau: the 32^4 headline run (default); coulomb: 32^3x64 all slices at once; "
@@ -394,6 +395,16 @@ def mag_line(args, L: int, maxiter: int, runs: str = "both") -> dict:
         reset()
         nf, msf, f = _run_chunked(lat, work, gf.MAG, args, prec, maxiter, stream, f"mag {L}^4 FP64")
         fp64 = {"time_to_theta_s": msf / 1e3, "sweeps": nf, "converged": f.converged, "gff": f.gff, "theta": f.theta}
+        fixed.insert(0, ("fp64", work))
+    if runs == "fp64-gauge":
+        # all-FP64 on the FP64 gauge-transformation sweep (gf_run_mixed
+        # GF_MIXED_FP64: the mixed run's FP64 phase with its check rules)
+        tuning_s = pretune(lat, src, gf.MAG, [gf.MIXED_FP64], stream)
+        reset()
+        nf, msf, f = _run_chunked(lat, work, gf.MAG, args, prec, maxiter, stream, f"mag {L}^4 FP64 gauge sweep",
+                                  gauge_sweep=True)
+        fp64 = {"time_to_theta_s": msf / 1e3, "sweeps": nf, "converged": f.converged, "gff": f.gff, "theta": f.theta,
+                "driver": "gf_run_mixed GF_MIXED_FP64"}
         fixed.insert(0, ("fp64", work))
     # U1xU1 on each MAG result
     u1 = {}

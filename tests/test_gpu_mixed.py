@@ -428,6 +428,43 @@ def test_run_mixed_mag(gf, gpu, oracle, mode):
                                                                            rel=1e-13)
 
 
+def test_mag_fp64_phase_coarse_checks_keep_the_sweep_count(gf, gpu):
+    """MAG's FP64 phase checks every 16 sweeps while theta, falling at the
+    rate of the host's reads, is more than 160 sweeps from the target.  On a
+    12^4 lattice (sigma 0.35) taken to theta < 1e-12 -- thousands of sweeps,
+    most of them with coarse checks -- the all-FP64 gauge-transformation run
+    still stops on the sweep count of run_fast, which checks every sweep
+    (+-1: the two round differently), and the returned links, measured again
+    by the drop-in quality kernel, meet the precision; so does the mixed run
+    (24-bit FP32 phase to 1e-6), on the same functional."""
+    dims = (12, 12, 12, 12)
+    lat = gf.Lattice(dims)
+    U = _cfg(gf, dims, sigma=0.35)
+
+    def theta_of(buf):
+        st = gf.GaugeFixingStats(gf.MAGKernelsSU3, buf, lat)
+        st.generateGaugeQuality()
+        t = st.getCurrentA()
+        st.close()
+        return t
+
+    d_fast = gf.DeviceBuffer.from_numpy(U)
+    fast = gf.run_fast(lat, d_fast, gf.MAG, 1.7, 1e-12, 200000, 100)
+    d_g64 = gf.DeviceBuffer.from_numpy(U)
+    g64 = gf.run_mixed(lat, d_g64, gf.MAG, 1.7, 1e-12, 1e-12, 200000, mode=gf.MIXED_FP64)
+    d_mix = gf.DeviceBuffer.from_numpy(U)
+    mix = gf.run_mixed(lat, d_mix, gf.MAG, 1.7, 1e-6, 1e-12, 200000, mode=gf.MIXED_GAUGE24)
+    assert fast.converged == g64.converged == mix.converged == 1
+    assert fast.sweeps > 1000
+    assert abs(g64.sweeps - fast.sweeps) <= 1, (g64.sweeps, fast.sweeps)
+    assert g64.checks < fast.sweeps / 4, (g64.checks, fast.sweeps)  # coarse checks took most of the run
+    for buf, r in ((d_fast, fast), (d_g64, g64), (d_mix, mix)):
+        assert r.theta < 1e-12
+        assert theta_of(buf) < 1e-12
+    assert g64.gff == pytest.approx(fast.gff, rel=1e-12)
+    assert mix.gff == pytest.approx(fast.gff, rel=1e-12)
+
+
 @pytest.mark.parametrize("gauge,prec", [(LANDAU, 1e-12), (COULOMB, 1e-12), (MAG, 1e-10), (U1XU1, 1e-30)])
 def test_fp64_gauge_sweep_matches_fp64_link_sweeps(gf, gpu, oracle, gauge, prec):
     """The all-FP64 run on the FP64 gauge-transformation sweep (run_mixed

@@ -1969,11 +1969,12 @@ gf_status gf_run_mixed(const gf_lattice *lat, void *U, int gauge, float omega, f
     // from the start (not after a stagnated phase, whose theta is further off:
     // the device sets `near` once a check comes close)
     const bool near = reached_switch && switch_precision < kNearFactor * precision;
-    // MAG's checks cost a third of a sweep in this phase (its theta is formed
-    // from the links in every wave); far from the target it converges by a few
-    // percent per hundred sweeps, so it checks every 16 sweeps while a theta
-    // the host has read is far from the precision (fused_loop's rules)
-    const int coarse64 = gauge == GF_MAG ? 16 : 0;
+    // a check costs the fused theta in the sweep kernels and a check kernel
+    // (MAG's a third of a sweep in this phase: its theta is formed from the
+    // links in every wave), so the phase checks every 16 sweeps while a theta
+    // the host has read is far from the precision (fused_loop's rules: MAG
+    // far from the target converges by a few percent per hundred sweeps)
+    const int coarse64 = 16;
     if (gauge64) {
         // FP64 sweeps of a second gauge transformation g64 (from 1) on the
         // links G U0 G^dagger, read only: the same OR updates as sweeping the

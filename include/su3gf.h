@@ -367,7 +367,9 @@ gf_status gf_block_order_choice(const gf_lattice *lat, int gauge, int mode, int 
  * goes on); then
  * U1 = G(x) U0_mu(x) G(x+mu)^dagger in FP64 with G the FP64 projection of g,
  * so FP32 rounding never alters the gauge orbit, and fused-check FP64 sweeps
- * continue to theta < precision, checked every sweep; the array then holds
+ * continue to theta < precision, checked every sweep once theta, falling at
+ * the rate the checks show, is within 160 sweeps of the precision, and every
+ * 16 sweeps before that (the same stopping sweep, fewer checks); the array then holds
  * the state that met it, as in gf_run_fast (with switch_precision <
  * 2 * precision the FP64 checks are made before the even update from the
  * first sweep on).  In the gauge-transformation modes (and GF_MIXED_FP64)
